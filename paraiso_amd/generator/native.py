@@ -1,0 +1,52 @@
+"""Code-generation setup (``Language/Paraiso/Generator/Native.hs:16-43``).
+
+The reference selects ``CPlusPlus`` or ``CUDA``; this backend adds ``HIP``,
+the MI355X path.  ``cuda_grid_size`` is kept for source compatibility and is
+read as a launch hint (workgroup tile) by the HIP generator.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+from enum import Enum
+from typing import Optional, Tuple
+
+from .. import annotation as A
+
+
+class Language(Enum):
+    CPlusPlus = "CPlusPlus"
+    CUDA = "CUDA"
+    HIP = "HIP"
+
+
+@dataclass
+class Setup:
+    local_size: Tuple[int, ...]
+    language: Language = Language.HIP
+    directory: str = "./"
+    opt_level: A.Level = A.Level.O3
+    boundary: Tuple[A.Condition, ...] = None
+    cuda_grid_size: Tuple[int, int] = (32, 32)
+    # HIP-only knobs (tile shape, temporal unroll); None lets the generator choose
+    tile: Optional[Tuple[int, ...]] = None
+    steps_per_launch: int = 1
+
+    def __post_init__(self):
+        self.local_size = tuple(int(x) for x in self.local_size)
+        if self.boundary is None:
+            self.boundary = tuple(A.Condition.Open for _ in self.local_size)
+        self.boundary = tuple(self.boundary)
+        if len(self.boundary) != len(self.local_size):
+            raise ValueError("boundary and local_size differ in dimension")
+
+    @property
+    def ndim(self) -> int:
+        return len(self.local_size)
+
+    def with_(self, **kw) -> "Setup":
+        return replace(self, **kw)
+
+
+def default_setup(local_size) -> Setup:
+    """``defaultSetup`` (Native.hs:98-107): Open boundaries, O3, grid (32,32)."""
+    return Setup(local_size=tuple(local_size), language=Language.HIP)
