@@ -1,0 +1,129 @@
+/* paraiso_hip.h -- C ABI of the MI355X (gfx950) Orthotope Machine runtime.
+ *
+ * This ABI replaces the class the reference generator emits for each
+ * Orthotope Machine (Language/Paraiso/Generator/PlanTrans.hs:50-276, the
+ * CS2/CS3 call stacks): construction, one call per OM kernel, accessors to the
+ * static variables, and the size queries.  The generated host code for a
+ * machine (one shared library per machine) exports a single descriptor,
+ * `pm_describe`, whose launch callbacks start that machine's HIP kernels;
+ * everything else -- device storage, ping-pong stores, scalar slots,
+ * reduction partials, streams, HIP graphs and the adjoint driver -- lives in
+ * this runtime.
+ *
+ * Conventions: every entry point returns 0 on success and a negative status
+ * on failure (PR_E_*), and never throws.  `pr_last_error` describes the last
+ * failure of a handle.  A handle is not thread-safe; different handles may be
+ * driven from different host threads and streams.  All work of a handle is
+ * issued on its stream (pr_set_stream; a fresh non-blocking stream of default
+ * priority otherwise).  Array statics are dense, x-fastest (index
+ * x + nx*(y + ny*z)), with no margins on Cyclic axes.
+ */
+#ifndef PARAISO_HIP_H
+#define PARAISO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PR_ABI_VERSION 1
+
+enum pr_status {
+  PR_OK = 0,
+  PR_E_INVALID = -1,     /* bad argument (index, size, pointer)          */
+  PR_E_HIP = -2,         /* a HIP runtime call failed                     */
+  PR_E_NOMEM = -3,       /* device allocation failed                      */
+  PR_E_ABI = -4,         /* descriptor ABI version mismatch               */
+  PR_E_UNSUPPORTED = -5  /* the machine does not provide the operation    */
+};
+
+enum pr_realm { PR_SCALAR = 0, PR_ARRAY = 1 };
+enum pr_ctype { PR_BOOL = 0, PR_INT = 1, PR_FLOAT = 2, PR_DOUBLE = 3 };
+
+typedef struct pr_machine pr_machine;
+
+/* What a generated launch callback receives (one kernel launch of one OM kernel). */
+typedef struct pr_launch_ctx {
+  const void* const* cur;  /* per static: current device buffer (arrays) or NULL */
+  void* const* next;       /* per static: buffer a store writes (stored arrays) or NULL */
+  char* scal;              /* device scalar block: statics, slots, counters */
+  char* part;              /* this launch's reduction partials */
+  unsigned* counter;       /* this launch's arrival counter (kept at 0 between launches) */
+  int n[3];                /* grid extent per axis (1 for absent axes) */
+  int tiles[3];            /* tiles per axis */
+  unsigned ntiles;         /* product of tiles[] */
+  void* stream;            /* hipStream_t */
+} pr_launch_ctx;
+
+typedef int (*pr_launch_fn)(const pr_launch_ctx* ctx);
+
+typedef struct pr_static_desc {
+  const char* name;
+  int realm;               /* pr_realm */
+  int ctype;               /* pr_ctype */
+} pr_static_desc;
+
+typedef struct pr_launch_desc {
+  pr_launch_fn fn;
+  int tile[3];             /* workgroup tile; 0 tiles = one workgroup (scalar-only launch) */
+  int n_reduce;            /* reductions finished by this launch's last workgroup */
+} pr_launch_desc;
+
+typedef struct pr_kernel_desc {
+  const char* name;
+  int n_launch;
+  const pr_launch_desc* launches;
+  int n_stored;            /* array statics this kernel stores (ping-pong flipped after it) */
+  const int* stored;
+} pr_kernel_desc;
+
+typedef struct pr_machine_desc {
+  int abi;                 /* PR_ABI_VERSION */
+  const char* name;
+  int ndim;
+  int n_static;
+  const pr_static_desc* statics;
+  int n_kernel;
+  const pr_kernel_desc* kernels;
+  int n_slot;              /* scalar slots after the statics in the scalar block */
+  int halo;                /* minimum grid extent per axis the kernels need */
+} pr_machine_desc;
+
+/* Exported by every generated machine library. */
+const pr_machine_desc* pm_describe(void);
+
+/* ---- lifetime (replaces the generated class constructor, PlanTrans.hs:75-86) */
+int pr_create(const pr_machine_desc* desc, const int* local_size, pr_machine** out);
+int pr_destroy(pr_machine* m);
+int pr_set_stream(pr_machine* m, void* hip_stream);
+int pr_sync(pr_machine* m);
+const char* pr_last_error(const pr_machine* m);
+
+/* ---- OM kernels (replaces `void <kernel>()`, PlanTrans.hs:225-276)
+ * pr_run issues `count` consecutive calls of kernel `k`; pr_run_graph does the
+ * same through a cached HIP graph of `count` calls. */
+int pr_run(pr_machine* m, int k, int count);
+int pr_run_graph(pr_machine* m, int k, int count);
+
+/* ---- statics (replaces `T& name()` / `T& name(i0,i1,...)`, PlanTrans.hs:117-157) */
+int pr_static_devptr(pr_machine* m, int s, void** dev_ptr);
+int pr_static_write(pr_machine* m, int s, const void* src, size_t bytes, int src_on_device);
+int pr_static_read(pr_machine* m, int s, void* dst, size_t bytes, int dst_on_device);
+size_t pr_static_bytes(const pr_machine* m, int s);
+
+/* ---- sizes (replace om_size, om_memory_size and friends, PlanTrans.hs:167-215) */
+int pr_size(const pr_machine* m, int axis);
+long long pr_cells(const pr_machine* m);
+
+/* ---- state snapshots for checkpointed adjoints (no counterpart in the reference)
+ * A snapshot holds copies of every array static and the scalar statics. */
+int pr_snapshot_alloc(pr_machine* m, int n_slots);
+int pr_snapshot_save(pr_machine* m, int slot);
+int pr_snapshot_load(pr_machine* m, int slot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARAISO_HIP_H */

@@ -334,9 +334,7 @@ def emit(plan: Plan, abs_mode: str = "fabs") -> str:
     for ki, k in enumerate(om.kernels):
         body = []
         g = k.graph
-        for s in plan.subkernels:
-            if s.kernel_idx != ki:
-                continue
+        for s in call_order(plan, ki):
             argnames = [_find_var(plan, ki, g, idx) for idx in s.inputs + s.outputs]
             body.append("%s(%s);" % (s.name, ", ".join(argnames)))
         for i, nd in enumerate(g.nodes):
@@ -373,6 +371,41 @@ def emit(plan: Plan, abs_mode: str = "fabs") -> str:
     lines.append("int oracle_memory_size() { return %d; }" % mem_total)
     lines.append("}")
     return "\n".join(lines) + "\n"
+
+
+def call_order(plan: Plan, ki: int) -> List[SubKernel]:
+    """Subkernel calls of kernel ``ki``: write-group order, except that a subkernel
+    never runs before one whose outputs it reads.
+
+    The reference calls subkernels in group order (PlanTrans.hs:233-241), but
+    its grouping (DependencyAnalysis.hs:111-117) can put a Manifest node in an
+    earlier group than a Manifest node it depends on: in the current Hydro,
+    ``broadcast (cast (loadSize axis))`` lands in group 0 while the cast it reads
+    is in group 1, so the reference's first ``proceed`` reads an uninitialised
+    scalar and later calls read the previous call's value.  Ordering by
+    dependency gives that steady-state behaviour from the first call on; it is
+    the identity order whenever the grouping is consistent.
+    """
+    subs = [s for s in plan.subkernels if s.kernel_idx == ki]
+    produced = {}
+    for s in subs:
+        for o in s.outputs:
+            produced[o] = s.group
+    done, order = set(), []
+    while len(order) < len(subs):
+        progressed = False
+        for s in subs:
+            if s.group in done:
+                continue
+            deps = set(produced[i] for i in s.inputs if i in produced and produced[i] != s.group)
+            if deps <= done:
+                order.append(s)
+                done.add(s.group)
+                progressed = True
+                break
+        if not progressed:
+            raise ValueError("cyclic subkernel dependencies in kernel %d" % ki)
+    return order
 
 
 def _find_var(plan: Plan, ki: int, g, idx: int) -> str:

@@ -1,0 +1,41 @@
+"""ORACLE -- test infrastructure only: compile the CPU machines the GPU parity
+tests use, so the GPU box does not spend its time in g++."""
+import os
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from fractions import Fraction
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from paraiso_amd import annotation as A                      # noqa: E402
+from paraiso_amd.generator.native import Language, Setup     # noqa: E402
+from paraiso_amd.generator.plan import translate             # noqa: E402
+from paraiso_amd.om.dyn import CType                         # noqa: E402
+
+from . import cpp_emitter                                    # noqa: E402
+
+G = Fraction(7, 5)
+
+
+def _cases():
+    from paraiso_amd.schemes.life import life_om
+    from paraiso_amd.schemes.hydro import hydro_om
+    cyc2 = (A.Condition.Cyclic,) * 2
+    out = []
+    for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96)]:
+        out.append((life_om, n, cyc2))
+    for n in [(64, 48), (96, 64), (48, 32)]:
+        out.append((lambda: hydro_om(CType.Double, G), n, cyc2))
+        out.append((lambda: hydro_om(CType.Float, G), n, cyc2))
+    return out
+
+
+def prebuild(jobs: int = 4):
+    plans = [translate(Setup(n, Language.CPlusPlus, boundary=b), f()) for f, n, b in _cases()]
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda p: cpp_emitter.build(p), plans))
+
+
+if __name__ == "__main__":
+    prebuild()

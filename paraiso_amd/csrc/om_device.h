@@ -1,0 +1,86 @@
+// Hand-written device templates shared by every generated OM machine (gfx950).
+//
+// The generator instantiates these for each kernel of an Orthotope Machine:
+//   * om_wrap        cyclic index wrap for halo reads
+//                    (the reference's `(x + c + n) % n`, Generator/PlanTrans.hs:459-461,481-483),
+//   * om_max/om_min  exactly std::max/std::min (the reference's CPU Max/Min,
+//                    Generator/PlanTrans.hs:690-691): ties and NaNs resolve as there,
+//   * om_block_*     wave64 shuffle tree + LDS combine for Reduce, and the
+//                    last-workgroup finish that replaces the reference's serial
+//                    om_reduce_* (Generator/PlanTrans.hs:719) with a deterministic
+//                    fixed-order tree over per-workgroup partials.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef OM_NT
+#define OM_NT 256
+#endif
+#define OM_NWAVES (OM_NT / 64)
+
+__device__ __forceinline__ int om_wrap(int x, int n) {
+  // one conditional add/subtract covers -n <= x < 2n; tiny grids fall back to %
+  x = (x < 0) ? x + n : x;
+  x = (x >= n) ? x - n : x;
+  if ((unsigned)x >= (unsigned)n) x = ((x % n) + n) % n;
+  return x;
+}
+
+template <class T> __device__ __forceinline__ T om_max(T a, T b) { return (a < b) ? b : a; }
+template <class T> __device__ __forceinline__ T om_min(T a, T b) { return (b < a) ? b : a; }
+
+struct OmSum {
+  template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OmSumWrap {   // integer sum with two's-complement wrap (order independent)
+  __device__ __forceinline__ int operator()(int a, int b) const {
+    return (int)((unsigned)a + (unsigned)b);
+  }
+};
+struct OmMin {
+  template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return om_min(a, b); }
+};
+struct OmMax {
+  template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return om_max(a, b); }
+};
+
+// Fixed-shape reduction of one value per thread to lane 0 of wave 0.
+// The xor-shuffle tree and the wave order are fixed, so results are
+// bit-reproducible run to run.
+template <class T, class Op>
+__device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= OM_NWAVES */) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  T r = scratch[0];
+#pragma unroll
+  for (int w = 1; w < OM_NWAVES; ++w) r = op(r, scratch[w]);
+  return r;   // every thread gets the block result
+}
+
+// Reduce `count` partials in a fixed order: thread t folds t, t+NT, ... then the tree.
+template <class T, class Op>
+__device__ __forceinline__ T om_partials_reduce(const T* __restrict__ part, int count, T init,
+                                                 Op op, T* scratch) {
+  T acc = init;
+  for (int i = threadIdx.x; i < count; i += OM_NT) acc = op(acc, part[i]);
+  return om_block_reduce(acc, op, scratch);
+}
+
+// Publish this workgroup's partials and learn whether it is the last one to finish.
+// The last workgroup resets the arrival counter, so the kernel can be replayed.
+__device__ __forceinline__ bool om_arrive_last(unsigned* counter, unsigned nblocks, int* flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    *flag = (prev == nblocks - 1u);
+  }
+  __syncthreads();
+  const bool last = *flag != 0;
+  if (last) __threadfence();
+  return last;
+}

@@ -1,0 +1,384 @@
+// Orthotope Machine runtime for MI355X: the hand-written side of the C ABI in
+// include/paraiso_hip.h.  It owns device storage and executes the launch
+// sequence a generated machine describes.
+//
+// Storage model (replacing the reference's std::vector/thrust_vector members
+// and deep-copy stores, Generator/PlanTrans.hs:75-86,243-258):
+//   * every Array static has one device buffer, plus a second one when some
+//     kernel stores it; a store writes the second buffer and the pair is
+//     swapped after the kernel, so every launch of the kernel still reads the
+//     old value, as the reference's copy-after-all-subkernels does;
+//   * Scalar statics, tail-computed scalar slots and arrival counters share
+//     one small device block (8 bytes per entry), so a step never returns
+//     to the host;
+//   * reduction partials live in one buffer sized from the tile counts.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <array>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "paraiso_hip.h"
+
+struct GraphKey {
+  int kernel, count;
+  unsigned long long parity;
+  bool operator<(const GraphKey& o) const {
+    if (kernel != o.kernel) return kernel < o.kernel;
+    if (count != o.count) return count < o.count;
+    return parity < o.parity;
+  }
+};
+
+struct pr_machine {
+  const pr_machine_desc* d = nullptr;
+  int n[3] = {1, 1, 1};
+  long long cells = 1;
+  std::vector<void*> buf[2];        // per static
+  std::vector<int> cur;             // parity per static
+  std::vector<char> stored_any;     // static stored by some kernel
+  char* scal = nullptr;
+  size_t scal_bytes = 0;
+  char* part = nullptr;
+  std::vector<std::vector<size_t>> part_off;    // [kernel][launch]
+  std::vector<std::vector<unsigned>> ntiles;    // [kernel][launch]
+  std::vector<std::vector<std::array<int, 3>>> tiles;
+  std::vector<std::vector<size_t>> counter_idx; // [kernel][launch]
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::vector<std::vector<void*>> snaps;        // snapshot buffers
+  std::vector<char*> snap_scal;
+  std::string err;
+};
+
+namespace {
+
+size_t ctype_size(int ct) {
+  switch (ct) {
+    case PR_BOOL: return 1;
+    case PR_INT: return 4;
+    case PR_FLOAT: return 4;
+    case PR_DOUBLE: return 8;
+  }
+  return 8;
+}
+
+int fail(pr_machine* m, int code, const std::string& msg) {
+  if (m) m->err = msg;
+  return code;
+}
+
+int hip_fail(pr_machine* m, hipError_t e, const char* what) {
+  return fail(m, PR_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PR_HIP(m, call)                                    \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail((m), e_, #call); \
+  } while (0)
+
+unsigned long long parity_mask(const pr_machine* m) {
+  unsigned long long p = 0;
+  for (size_t s = 0; s < m->cur.size() && s < 64; ++s) p |= (unsigned long long)(m->cur[s] & 1) << s;
+  return p;
+}
+
+int launch_kernel_once(pr_machine* m, int k) {
+  const pr_kernel_desc& kd = m->d->kernels[k];
+  const int ns = m->d->n_static;
+  std::vector<const void*> cur(ns, nullptr);
+  std::vector<void*> next(ns, nullptr);
+  for (int s = 0; s < ns; ++s) {
+    if (m->d->statics[s].realm != PR_ARRAY) continue;
+    cur[s] = m->buf[m->cur[s]][s];
+    if (m->stored_any[s]) next[s] = m->buf[1 - m->cur[s]][s];
+  }
+  for (int l = 0; l < kd.n_launch; ++l) {
+    pr_launch_ctx ctx;
+    ctx.cur = cur.data();
+    ctx.next = next.data();
+    ctx.scal = m->scal;
+    ctx.part = m->part + m->part_off[k][l];
+    ctx.counter = reinterpret_cast<unsigned*>(m->scal + 8 * m->counter_idx[k][l]);
+    for (int a = 0; a < 3; ++a) {
+      ctx.n[a] = m->n[a];
+      ctx.tiles[a] = m->tiles[k][l][a];
+    }
+    ctx.ntiles = m->ntiles[k][l];
+    ctx.stream = m->stream;
+    int rc = kd.launches[l].fn(&ctx);
+    if (rc != 0) {
+      hipError_t e = hipGetLastError();
+      return fail(m, PR_E_HIP, std::string("launch ") + kd.name + "[" + std::to_string(l) +
+                                   "]: " + hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+  return PR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pr_create(const pr_machine_desc* d, const int* local_size, pr_machine** out) {
+  if (!d || !local_size || !out) return PR_E_INVALID;
+  if (d->abi != PR_ABI_VERSION) return PR_E_ABI;
+  *out = nullptr;
+  pr_machine* m = new pr_machine();
+  m->d = d;
+  for (int a = 0; a < d->ndim; ++a) {
+    if (local_size[a] < 1 || local_size[a] < d->halo) {
+      int rc = fail(m, PR_E_INVALID, "grid extent " + std::to_string(local_size[a]) +
+                                          " on axis " + std::to_string(a) +
+                                          " is below the machine's halo " + std::to_string(d->halo));
+      delete m;
+      return rc;
+    }
+    m->n[a] = local_size[a];
+    m->cells *= local_size[a];
+  }
+  if (m->cells >= (1LL << 31)) {
+    delete m;
+    return PR_E_INVALID;
+  }
+  const int ns = d->n_static;
+  m->cur.assign(ns, 0);
+  m->stored_any.assign(ns, 0);
+  m->buf[0].assign(ns, nullptr);
+  m->buf[1].assign(ns, nullptr);
+  for (int k = 0; k < d->n_kernel; ++k)
+    for (int i = 0; i < d->kernels[k].n_stored; ++i) m->stored_any[d->kernels[k].stored[i]] = 1;
+  // scalar block: statics, slots, counters (8 bytes each)
+  size_t n_counter = 0;
+  m->counter_idx.resize(d->n_kernel);
+  for (int k = 0; k < d->n_kernel; ++k)
+    for (int l = 0; l < d->kernels[k].n_launch; ++l)
+      m->counter_idx[k].push_back(ns + d->n_slot + n_counter++);
+  m->scal_bytes = 8 * (ns + d->n_slot + n_counter);
+  // tiles and partials
+  size_t part_bytes = 0;
+  m->part_off.resize(d->n_kernel);
+  m->ntiles.resize(d->n_kernel);
+  m->tiles.resize(d->n_kernel);
+  for (int k = 0; k < d->n_kernel; ++k) {
+    for (int l = 0; l < d->kernels[k].n_launch; ++l) {
+      const pr_launch_desc& ld = d->kernels[k].launches[l];
+      std::array<int, 3> t = {1, 1, 1};
+      unsigned nt = 1;
+      if (ld.tile[0] > 0) {
+        for (int a = 0; a < 3; ++a) {
+          int ta = ld.tile[a] > 0 ? ld.tile[a] : 1;
+          t[a] = (m->n[a] + ta - 1) / ta;
+          nt *= (unsigned)t[a];
+        }
+      }
+      m->tiles[k].push_back(t);
+      m->ntiles[k].push_back(nt);
+      m->part_off[k].push_back(part_bytes);
+      part_bytes += ((size_t)ld.n_reduce * nt * 8 + 255) / 256 * 256;
+    }
+  }
+  hipError_t e;
+  e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    int rc = hip_fail(m, e, "hipStreamCreateWithFlags");
+    delete m;
+    return rc;
+  }
+  m->own_stream = true;
+  for (int s = 0; s < ns; ++s) {
+    if (d->statics[s].realm != PR_ARRAY) continue;
+    size_t bytes = (size_t)m->cells * ctype_size(d->statics[s].ctype);
+    for (int b = 0; b < (m->stored_any[s] ? 2 : 1); ++b) {
+      e = hipMalloc(&m->buf[b][s], bytes);
+      if (e == hipSuccess) e = hipMemsetAsync(m->buf[b][s], 0, bytes, m->stream);
+      if (e != hipSuccess) {
+        int rc = fail(m, PR_E_NOMEM, "hipMalloc of static buffer failed");
+        pr_destroy(m);
+        return rc;
+      }
+    }
+  }
+  e = hipMalloc(&m->scal, m->scal_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(m->scal, 0, m->scal_bytes, m->stream);
+  if (e == hipSuccess) e = hipMalloc(&m->part, part_bytes > 0 ? part_bytes : 256);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    pr_destroy(m);
+    return PR_E_NOMEM;
+  }
+  *out = m;
+  return PR_OK;
+}
+
+int pr_destroy(pr_machine* m) {
+  if (!m) return PR_E_INVALID;
+  if (m->stream) hipStreamSynchronize(m->stream);
+  for (auto& kv : m->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (int b = 0; b < 2; ++b)
+    for (void* p : m->buf[b])
+      if (p) (void)hipFree(p);
+  for (auto& v : m->snaps)
+    for (void* p : v)
+      if (p) (void)hipFree(p);
+  for (char* p : m->snap_scal)
+    if (p) (void)hipFree(p);
+  if (m->scal) hipFree(m->scal);
+  if (m->part) hipFree(m->part);
+  if (m->own_stream && m->stream) hipStreamDestroy(m->stream);
+  delete m;
+  return PR_OK;
+}
+
+int pr_set_stream(pr_machine* m, void* stream) {
+  if (!m) return PR_E_INVALID;
+  PR_HIP(m, hipStreamSynchronize(m->stream));
+  if (m->own_stream) hipStreamDestroy(m->stream);
+  m->stream = static_cast<hipStream_t>(stream);
+  m->own_stream = false;
+  for (auto& kv : m->graphs) (void)hipGraphExecDestroy(kv.second);
+  m->graphs.clear();
+  return PR_OK;
+}
+
+int pr_sync(pr_machine* m) {
+  if (!m) return PR_E_INVALID;
+  PR_HIP(m, hipStreamSynchronize(m->stream));
+  return PR_OK;
+}
+
+const char* pr_last_error(const pr_machine* m) { return m ? m->err.c_str() : "null handle"; }
+
+int pr_run(pr_machine* m, int k, int count) {
+  if (!m || k < 0 || k >= m->d->n_kernel || count < 0) return PR_E_INVALID;
+  for (int c = 0; c < count; ++c) {
+    int rc = launch_kernel_once(m, k);
+    if (rc) return rc;
+  }
+  return PR_OK;
+}
+
+int pr_run_graph(pr_machine* m, int k, int count) {
+  if (!m || k < 0 || k >= m->d->n_kernel || count < 1) return PR_E_INVALID;
+  GraphKey key{k, count, parity_mask(m)};
+  auto it = m->graphs.find(key);
+  if (it == m->graphs.end()) {
+    std::vector<int> saved = m->cur;
+    hipGraph_t g;
+    PR_HIP(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PR_OK;
+    for (int c = 0; c < count && rc == PR_OK; ++c) rc = launch_kernel_once(m, k);
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    m->cur = saved;
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(m, e, "hipStreamEndCapture");
+    hipGraphExec_t ex;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(m, e, "hipGraphInstantiate");
+    it = m->graphs.emplace(key, ex).first;
+  }
+  PR_HIP(m, hipGraphLaunch(it->second, m->stream));
+  const pr_kernel_desc& kd = m->d->kernels[k];
+  if (count % 2)
+    for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+  return PR_OK;
+}
+
+size_t pr_static_bytes(const pr_machine* m, int s) {
+  if (!m || s < 0 || s >= m->d->n_static) return 0;
+  const pr_static_desc& sd = m->d->statics[s];
+  if (sd.realm == PR_ARRAY) return (size_t)m->cells * ctype_size(sd.ctype);
+  return ctype_size(sd.ctype);
+}
+
+int pr_static_devptr(pr_machine* m, int s, void** p) {
+  if (!m || !p || s < 0 || s >= m->d->n_static) return PR_E_INVALID;
+  if (m->d->statics[s].realm == PR_ARRAY)
+    *p = m->buf[m->cur[s]][s];
+  else
+    *p = m->scal + 8 * s;
+  return PR_OK;
+}
+
+int pr_static_write(pr_machine* m, int s, const void* src, size_t bytes, int src_on_device) {
+  if (!m || !src || s < 0 || s >= m->d->n_static) return PR_E_INVALID;
+  if (bytes != pr_static_bytes(m, s)) return fail(m, PR_E_INVALID, "static write: size mismatch");
+  void* dst;
+  pr_static_devptr(m, s, &dst);
+  PR_HIP(m, hipMemcpyAsync(dst, src, bytes, src_on_device ? hipMemcpyDeviceToDevice
+                                                          : hipMemcpyHostToDevice, m->stream));
+  if (!src_on_device) PR_HIP(m, hipStreamSynchronize(m->stream));
+  return PR_OK;
+}
+
+int pr_static_read(pr_machine* m, int s, void* dst, size_t bytes, int dst_on_device) {
+  if (!m || !dst || s < 0 || s >= m->d->n_static) return PR_E_INVALID;
+  if (bytes != pr_static_bytes(m, s)) return fail(m, PR_E_INVALID, "static read: size mismatch");
+  void* src;
+  pr_static_devptr(m, s, &src);
+  PR_HIP(m, hipMemcpyAsync(dst, src, bytes, dst_on_device ? hipMemcpyDeviceToDevice
+                                                          : hipMemcpyDeviceToHost, m->stream));
+  if (!dst_on_device) PR_HIP(m, hipStreamSynchronize(m->stream));
+  return PR_OK;
+}
+
+int pr_size(const pr_machine* m, int axis) {
+  if (!m || axis < 0 || axis >= 3) return 0;
+  return m->n[axis];
+}
+
+long long pr_cells(const pr_machine* m) { return m ? m->cells : 0; }
+
+int pr_snapshot_alloc(pr_machine* m, int n_slots) {
+  if (!m || n_slots < 0) return PR_E_INVALID;
+  const int ns = m->d->n_static;
+  while ((int)m->snaps.size() < n_slots) {
+    std::vector<void*> v(ns, nullptr);
+    for (int s = 0; s < ns; ++s) {
+      if (m->d->statics[s].realm != PR_ARRAY) continue;
+      if (hipMalloc(&v[s], pr_static_bytes(m, s)) != hipSuccess) {
+        for (void* p : v)
+          if (p) (void)hipFree(p);
+        return fail(m, PR_E_NOMEM, "snapshot allocation failed");
+      }
+    }
+    char* sc = nullptr;
+    if (hipMalloc(&sc, 8 * ns) != hipSuccess) return fail(m, PR_E_NOMEM, "snapshot scalars");
+    m->snaps.push_back(v);
+    m->snap_scal.push_back(sc);
+  }
+  return PR_OK;
+}
+
+int pr_snapshot_save(pr_machine* m, int slot) {
+  if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
+  const int ns = m->d->n_static;
+  for (int s = 0; s < ns; ++s) {
+    if (m->d->statics[s].realm != PR_ARRAY) continue;
+    PR_HIP(m, hipMemcpyAsync(m->snaps[slot][s], m->buf[m->cur[s]][s], pr_static_bytes(m, s),
+                             hipMemcpyDeviceToDevice, m->stream));
+  }
+  PR_HIP(m, hipMemcpyAsync(m->snap_scal[slot], m->scal, 8 * ns, hipMemcpyDeviceToDevice, m->stream));
+  return PR_OK;
+}
+
+int pr_snapshot_load(pr_machine* m, int slot) {
+  if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
+  const int ns = m->d->n_static;
+  for (int s = 0; s < ns; ++s) {
+    if (m->d->statics[s].realm != PR_ARRAY) continue;
+    PR_HIP(m, hipMemcpyAsync(m->buf[m->cur[s]][s], m->snaps[slot][s], pr_static_bytes(m, s),
+                             hipMemcpyDeviceToDevice, m->stream));
+  }
+  PR_HIP(m, hipMemcpyAsync(m->scal, m->snap_scal[slot], 8 * ns, hipMemcpyDeviceToDevice, m->stream));
+  return PR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""Code generation entry points (``Language/Paraiso/Generator.hs:29-66``).
+
+``generate(setup, om)`` returns ``[(file name, text)]`` and ``generate_io``
+writes them under ``setup.directory``, as in the reference.  The supported
+language is ``Language.HIP``: a ``<Name>.hip`` with the machine's kernels and
+its ``pm_describe`` descriptor, and a ``<Name>.hpp`` C++ class with the
+reference's interface (constructor, one method per kernel, accessors, size
+queries) that drives the machine through ``include/paraiso_hip.h``.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+from typing import List, Optional, Tuple
+
+from .. import annotation as A
+from ..om.graph import OM
+from .native import Language, Setup, default_setup
+from .plan import Plan, translate
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+CSRC_DIR = os.path.join(PKG_DIR, "csrc")
+INCLUDE_DIR = os.path.join(REPO_DIR, "include")
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+GEN_DIR = os.path.join(PKG_DIR, "_gen")
+
+DEFAULT_TILES = {1: (256,), 2: (64, 8), 3: (64, 4, 4)}
+
+HIPCC_BASE = ["hipcc", "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wno-unused-variable", "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def _hip_sources(setup: Setup, om: OM) -> Tuple[Plan, str, str]:
+    from .hip.emit import MachineEmitter
+    from .hip.host import emit_host_class
+    from .hip.ir import lower_kernel
+    from .hip.schedule import schedule_kernel
+    if any(b is not A.Condition.Cyclic for b in setup.boundary):
+        raise NotImplementedError("the HIP backend lowers Cyclic boundaries; Open margins are not "
+                                  "implemented for the GPU path")
+    plan = translate(setup, om)
+    om_opt = plan.om
+    tile = tuple(setup.tile) if setup.tile else DEFAULT_TILES[om_opt.ndim]
+    cyclic = tuple(b is A.Condition.Cyclic for b in setup.boundary)
+    scheds = [schedule_kernel(lower_kernel(om_opt, ki, cyclic), tile)
+              for ki in range(len(om_opt.kernels))]
+    src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile).emit()
+    hpp = emit_host_class(om_opt, setup)
+    return plan, src, hpp
+
+
+def generate(setup: Setup, om: OM) -> List[Tuple[str, str]]:
+    if setup.language is not Language.HIP:
+        raise NotImplementedError("this backend generates Language.HIP only")
+    _, src, hpp = _hip_sources(setup, om)
+    return [(om.name + ".hip", src), (om.name + ".hpp", hpp)]
+
+
+def generate_io(setup: Setup, om: OM) -> List[Tuple[str, str]]:
+    out = []
+    os.makedirs(setup.directory, exist_ok=True)
+    for fn, text in generate(setup, om):
+        path = os.path.join(setup.directory, fn)
+        with open(path, "w") as f:
+            f.write(text)
+        out.append((path, text))
+    return out
+
+
+def runtime_library() -> str:
+    return os.path.join(LIB_DIR, "libparaiso_hip.so")
+
+
+def build_runtime(force: bool = True) -> str:
+    """Compile the hand-written runtime into ``paraiso_amd/lib/libparaiso_hip.so``."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    so = runtime_library()
+    src = os.path.join(CSRC_DIR, "pr_runtime.cpp")
+    if not force and os.path.exists(so) and os.path.getmtime(so) >= os.path.getmtime(src):
+        return so
+    tmp = so + ".tmp%d" % os.getpid()
+    cmd = HIPCC_BASE + ["-I" + INCLUDE_DIR, "-x", "hip", src, "-o", tmp]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, so)
+    return so
+
+
+def build_machine(setup: Setup, om: OM, variant: str, precise: bool = False,
+                  force: bool = False) -> str:
+    """Generate and compile a machine into ``paraiso_amd/_gen/<variant>/lib<variant>.so``.
+
+    ``precise`` compiles without FP contraction and with correctly rounded
+    FP32 division and square root (bit-comparable with the CPU oracle);
+    the default contracts multiply-adds and uses the fast FP32 reciprocal.
+    """
+    _, src, hpp = _hip_sources(setup, om)
+    flags = list(HIPCC_BASE) + ["-I" + CSRC_DIR, "-I" + INCLUDE_DIR]
+    if precise:
+        flags += ["-ffp-contract=off"]
+    else:
+        flags += ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+    d = os.path.join(GEN_DIR, variant)
+    os.makedirs(d, exist_ok=True)
+    hip_path = os.path.join(d, variant + ".hip")
+    so = os.path.join(d, "lib%s.so" % variant)
+    key = hashlib.sha1((src + " ".join(flags) + open(os.path.join(CSRC_DIR, "om_device.h")).read()
+                        ).encode()).hexdigest()
+    stamp = os.path.join(d, "build.key")
+    if (not force and os.path.exists(so) and os.path.exists(stamp)
+            and open(stamp).read() == key):
+        return so
+    with open(hip_path, "w") as f:
+        f.write(src)
+    with open(os.path.join(d, om.name + ".hpp"), "w") as f:
+        f.write(hpp)
+    tmp = so + ".tmp%d" % os.getpid()
+    subprocess.run(flags + ["-x", "hip", hip_path, "-o", tmp], check=True)
+    os.replace(tmp, so)
+    with open(stamp, "w") as f:
+        f.write(key)
+    return so
+
+
+__all__ = ["Setup", "Language", "default_setup", "generate", "generate_io", "translate",
+           "build_machine", "build_runtime", "runtime_library"]

@@ -1,0 +1,432 @@
+"""HIP source emitter for scheduled OM kernels (gfx950).
+
+For every launch of every OM kernel it writes one ``__global__`` function built
+from the hand-written templates in ``csrc/om_device.h``:
+
+1. tile coordinates from the workgroup id, remapped so that each of the
+   eight XCDs owns a contiguous band of tiles (neighbouring tiles share halo
+   data in that XCD's L2),
+2. uniform scalars: scalar statics and slots read once per workgroup, and
+   the scalar arithmetic of level 0 (the reference's Scalar subkernels,
+   ``PlanTrans.hs:417``) evaluated redundantly in registers,
+3. one cooperative loop per phase over the phase's box, with coalesced
+   x-fastest addressing; LDS receives the materialised values,
+4. the store phase over the tile: stores go to the ping-pong "next" buffers,
+   Reduce arguments fold into per-thread accumulators (FP64 for floating
+   sums),
+5. the reduction epilogue: wave64 shuffle tree, per-workgroup partials, and
+   the last workgroup to arrive finishes every Reduce in a fixed order, runs
+   the scalar tail (``om_reduce_*``/``om_broadcast``, ``PlanTrans.hs:566-569``)
+   and, in the kernel's last launch, commits the scalar stores.
+
+Per-point expressions follow the reference's SASA naming (``a<node>_<cursor>``,
+``PlanTrans.hs:625-644``) as ``n<value>_<cursor>``.
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Set, Tuple
+
+from ...om.arith import Op, ReduceOp
+from ...om.dyn import CType, Realm
+from ...om.literal import c_literal
+from .ir import LKernel
+from .schedule import OUT, ArrayLaunch, KernelSchedule
+
+NT = 256
+PR_CTYPE = {CType.Bool: "PR_BOOL", CType.Int: "PR_INT", CType.Float: "PR_FLOAT",
+            CType.Double: "PR_DOUBLE"}
+
+
+def _cname(c: Tuple[int, ...]) -> str:
+    return "_".join(("m%d" % -x) if x < 0 else str(x) for x in c)
+
+
+def _p(x: str) -> str:
+    return "(" + x + ")"
+
+
+def acc_type(op: ReduceOp, ct: CType) -> CType:
+    if op is ReduceOp.Sum and ct.is_float:
+        return CType.Double
+    if ct is CType.Bool:
+        return CType.Int
+    return ct
+
+
+def acc_init(op: ReduceOp, ct: CType) -> str:
+    if op is ReduceOp.Sum:
+        return "0" if ct is CType.Int else "0.0"
+    big = {CType.Int: ("2147483647", "(-2147483647-1)"),
+           CType.Float: ("INFINITY", "-INFINITY"),
+           CType.Double: ("((double)INFINITY)", "(-(double)INFINITY)")}[ct]
+    return big[0] if op is ReduceOp.Min else big[1]
+
+
+def acc_op(op: ReduceOp, ct: CType) -> str:
+    if op is ReduceOp.Sum:
+        return "OmSumWrap()" if ct is CType.Int else "OmSum()"
+    return "OmMin()" if op is ReduceOp.Min else "OmMax()"
+
+
+def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType) -> str:
+    infix = {Op.Add: "+", Op.Sub: "-", Op.Mul: "*", Op.Div: "/", Op.Mod: "%",
+             Op.And: "&&", Op.Or: "||", Op.EQ: "==", Op.NE: "!=", Op.LT: "<",
+             Op.LE: "<=", Op.GT: ">", Op.GE: ">="}
+    t0 = arg_types[0] if arg_types else out
+    if op is Op.Identity:
+        return args[0]
+    if op in infix:
+        return "%s %s %s" % (_p(args[0]), infix[op], _p(args[1]))
+    if op is Op.Neg:
+        return "-" + _p(args[0])
+    if op is Op.Inv:
+        return "1/" + _p(args[0])
+    if op is Op.Not:
+        return "!" + _p(args[0])
+    if op is Op.Select:
+        return "%s ? %s : %s" % (_p(args[0]), _p(args[1]), _p(args[2]))
+    if op is Op.Max:
+        return "om_max<%s>(%s, %s)" % (out.c_name, args[0], args[1])
+    if op is Op.Min:
+        return "om_min<%s>(%s, %s)" % (out.c_name, args[0], args[1])
+    if op is Op.Abs:
+        if t0 is CType.Float:
+            return "fabsf(%s)" % args[0]
+        if t0 is CType.Double:
+            return "fabs(%s)" % args[0]
+        return "abs(%s)" % args[0]
+    if op is Op.Signum:
+        return "(%s)(((%s) > 0) - ((%s) < 0))" % (out.c_name, args[0], args[0])
+    if op is Op.Sqrt:
+        return ("sqrtf(%s)" if t0 is CType.Float else "sqrt(%s)") % args[0]
+    if op in (Op.Exp, Op.Log, Op.Sin, Op.Cos, Op.Tan, Op.Asin, Op.Acos, Op.Atan):
+        f = op.value.lower()
+        if t0 is CType.Float:
+            # the reference calls the double libm function on float operands
+            return "(float)%s((double)(%s))" % (f, args[0])
+        return "%s(%s)" % (f, args[0])
+    if op is Op.Atan2:
+        if t0 is CType.Float:
+            return "(float)atan2((double)(%s), (double)(%s))" % (args[0], args[1])
+        return "atan2(%s, %s)" % (args[0], args[1])
+    if op is Op.Cast:
+        return "(%s)(%s)" % (cast_to.c_name, args[0])
+    raise NotImplementedError("operator %s has no HIP lowering" % op.value)
+
+
+class MachineEmitter:
+    def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
+                 tile: Tuple[int, ...]):
+        self.name = name
+        self.statics = statics           # list of Named
+        self.ndim = ndim
+        self.schedules = schedules
+        self.tile = tuple(tile) + (1,) * (3 - len(tile))
+        self.ns = len(statics)
+        # slot numbering across kernels
+        self.slot_index: Dict[Tuple[int, int], int] = {}
+        for ki, sch in enumerate(schedules):
+            for v in sch.slot_scalars:
+                self.slot_index[(ki, v)] = len(self.slot_index)
+
+    # ------------------------------------------------------------ helpers
+    def slot_off(self, ki, v) -> int:
+        return 8 * (self.ns + self.slot_index[(ki, v)])
+
+    def scalar_expr(self, k: LKernel, v: int) -> str:
+        n = k.nodes[v]
+        if n.op == "load":
+            return "*(const %s*)(a.scal + %d)" % (n.ctype.c_name, 8 * n.payload)
+        if n.op == "imm":
+            return c_literal(n.payload, n.ctype)
+        if n.op == "size":
+            return "(%s)(a.n%d)" % (n.ctype.c_name, n.payload)
+        if n.op == "arith":
+            op, cast_to = n.payload
+            return device_arith(op, ["u%d" % x for x in n.args], cast_to,
+                                [k.nodes[x].ctype for x in n.args], n.ctype)
+        raise NotImplementedError("scalar %s" % n.op)
+
+    # ------------------------------------------------------------ per launch
+    def emit_launch(self, ki: int, sch: KernelSchedule, la: ArrayLaunch) -> Tuple[str, str]:
+        k = sch.kernel
+        nodes = k.nodes
+        nd = self.ndim
+        T = self.tile
+        fn = "om_k%d_l%d" % (ki, la.index)
+        has_work = bool(la.targets)
+        L = la.index
+        # ---- argument struct
+        loads = sorted(set(nodes[v].payload for v in la.lc if nodes[v].op == "load"
+                           and nodes[v].realm is Realm.Array))
+        stores = sorted(set(s for s, _ in la.stores))
+        fields = []
+        for s in loads:
+            fields.append("const %s* __restrict__ ld%d;" % (self.statics[s].dyn.ctype.c_name, s))
+        for s in stores:
+            fields.append("%s* __restrict__ st%d;" % (self.statics[s].dyn.ctype.c_name, s))
+        fields += ["char* __restrict__ scal;", "char* __restrict__ part;", "unsigned* counter;",
+                   "int n0, n1, n2;", "int t0, t1, t2;", "unsigned ntiles;"]
+        struct = "struct Args_%s {\n  %s\n};\n" % (fn, "\n  ".join(fields))
+
+        body: List[str] = []
+        w = body.append
+        w("extern \"C\" __global__ void __launch_bounds__(%d) %s(Args_%s a) {" % (NT, fn, fn))
+        if la.lds_bytes:
+            w("  __shared__ __align__(16) char smem[%d];" % la.lds_bytes)
+        w("  __shared__ int om_flag;")
+        acc_types = set()
+        for r in la.reduces:
+            acc_types.add(acc_type(nodes[r].payload, nodes[nodes[r].args[0]].ctype))
+        for at in sorted(acc_types, key=lambda c: c.value):
+            w("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
+        # tile coordinates
+        w("  unsigned tile = blockIdx.x;")
+        w("  if ((a.ntiles & 7u) == 0u) tile = (tile & 7u) * (a.ntiles >> 3) + (tile >> 3);")
+        w("  const int X0 = (int)(tile %% (unsigned)a.t0) * %d;" % T[0])
+        w("  const int X1 = (int)((tile / (unsigned)a.t0) %% (unsigned)a.t1) * %d;" % T[1])
+        w("  const int X2 = (int)(tile / ((unsigned)a.t0 * (unsigned)a.t1)) * %d;" % T[2])
+        w("  (void)X0; (void)X1; (void)X2;")
+        # uniform scalars
+        for n in nodes:
+            if n.realm is not Realm.Scalar:
+                continue
+            lvl = sch.level[n.id]
+            if lvl == 0:
+                w("  const %s u%d = %s;" % (n.ctype.c_name, n.id, self.scalar_expr(k, n.id)))
+            elif lvl <= L:
+                w("  const %s u%d = *(const %s*)(a.scal + %d);" % (
+                    n.ctype.c_name, n.id, n.ctype.c_name, self.slot_off(ki, n.id)))
+        # LDS views
+        for v, b in sorted(la.lds.items()):
+            w("  %s* __restrict__ s%d = (%s*)(smem + %d);" % (
+                nodes[v].ctype.c_name, v, nodes[v].ctype.c_name, b.offset))
+        # accumulators
+        for r in la.reduces:
+            a_ct = nodes[nodes[r].args[0]].ctype
+            at = acc_type(nodes[r].payload, a_ct)
+            w("  %s acc%d = %s;" % (at.c_name, r, acc_init(nodes[r].payload, at)))
+        if has_work:
+            for ph in la.phases:
+                self._emit_phase(k, la, ph, w)
+        # ---- epilogue
+        needs_tail = bool(la.reduces) or bool(la.tail_scalars) or la.commit
+        if needs_tail:
+            if has_work:
+                for r in la.reduces:
+                    a_ct = nodes[nodes[r].args[0]].ctype
+                    at = acc_type(nodes[r].payload, a_ct)
+                    j = la.reduces.index(r)
+                    w("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
+                        at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
+                    w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + blockIdx.x] = br; }" % (
+                        at.c_name, j))
+                w("  if (om_arrive_last(a.counter, a.ntiles, &om_flag)) {")
+            else:
+                w("  {")
+            for j, r in enumerate(la.reduces):
+                a_ct = nodes[nodes[r].args[0]].ctype
+                at = acc_type(nodes[r].payload, a_ct)
+                src = "acc%d" % r if not has_work else "om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.ntiles, (%s)%s, %s, om_scr_%s)" % (
+                    at.c_name, j, at.c_name, acc_init(nodes[r].payload, at),
+                    acc_op(nodes[r].payload, at), at.c_name)
+                w("    const %s fr%d = %s;" % (at.c_name, r, src))
+            w("    if (threadIdx.x == 0) {")
+            for v in la.tail_scalars:
+                n = nodes[v]
+                if n.op == "reduce":
+                    expr = "(%s)(fr%d)" % (n.ctype.c_name, v)
+                else:
+                    expr = self.scalar_expr(k, v)
+                w("      const %s u%d = %s;" % (n.ctype.c_name, v, expr))
+                w("      *(%s*)(a.scal + %d) = u%d;" % (n.ctype.c_name, self.slot_off(ki, v), v))
+            if la.commit:
+                for s, v in sch.scalar_stores:
+                    w("      *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s, v))
+            if has_work:
+                w("      *a.counter = 0u;")
+            w("    }")
+            w("  }")
+        w("}")
+        kernel_src = struct + "\n".join(body) + "\n"
+
+        # ---- host launcher
+        h = []
+        h.append("static int launch_%s(const pr_launch_ctx* c) {" % fn)
+        h.append("  Args_%s a;" % fn)
+        for s in loads:
+            h.append("  a.ld%d = (const %s*)c->cur[%d];" % (s, self.statics[s].dyn.ctype.c_name, s))
+        for s in stores:
+            h.append("  a.st%d = (%s*)c->next[%d];" % (s, self.statics[s].dyn.ctype.c_name, s))
+        h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
+        h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
+        h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
+        h.append("  hipLaunchKernelGGL(%s, dim3(c->ntiles), dim3(%d), 0, (hipStream_t)c->stream, a);"
+                 % (fn, NT))
+        h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
+        h.append("}")
+        return kernel_src, "\n".join(h) + "\n"
+
+    def _emit_phase(self, k: LKernel, la: ArrayLaunch, ph, w):
+        nodes = k.nodes
+        nd = self.ndim
+        T = self.tile
+        box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
+        W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
+        vol = W[0] * W[1] * W[2]
+        is_out = ph.nodes == [OUT]
+        w("  // phase %d: %s over box %s" % (ph.index, "store" if is_out else
+                                             "values " + ",".join("n%d" % v for v in ph.nodes),
+                                             list(box[:nd])))
+        w("  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, NT))
+        w("    const int l0 = p %% %d;" % W[0])
+        w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
+        w("    const int l2 = p / %d;" % (W[0] * W[1]))
+        w("    (void)l0; (void)l1; (void)l2;")
+        for ax in range(3):
+            w("    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+        if is_out:
+            conds = ["g%d < a.n%d" % (ax, ax) for ax in range(nd)]
+            w("    if (!(%s)) continue;" % " && ".join(conds))
+            w("    const int addr0 = %s;" % self._addr(["g0", "g1", "g2"]))
+        roots = [t for t in la.targets if t not in la.mat] if is_out else list(ph.nodes)
+        ctx = OUT if is_out else None
+        lines = self._ctx_body(k, la, roots, box, W, is_out)
+        for ln in lines:
+            w("    " + ln)
+        zero = tuple(0 for _ in range(nd))
+        if is_out:
+            for s, v in la.stores:
+                w("    a.st%d[addr0] = %s;" % (s, self._val_name(k, la, v, zero, roots)))
+            for r in la.reduces:
+                arg = nodes[r].args[0]
+                at = acc_type(nodes[r].payload, nodes[arg].ctype)
+                w("    acc%d = %s(acc%d, (%s)%s);" % (r, acc_op(nodes[r].payload, at), r, at.c_name,
+                                                     self._val_name(k, la, arg, zero, roots)))
+        else:
+            for v in ph.nodes:
+                w("    s%d[p] = n%d_%s;" % (v, v, _cname(zero)))
+        w("  }")
+        w("  __syncthreads();")
+
+    def _val_name(self, k, la, v, c, roots):
+        return "n%d_%s" % (v, _cname(c))
+
+    def _addr(self, g: List[str]) -> str:
+        if self.ndim == 1:
+            return g[0]
+        if self.ndim == 2:
+            return "%s * a.n0 + %s" % (g[1], g[0])
+        return "(%s * a.n1 + %s) * a.n0 + %s" % (g[2], g[1], g[0])
+
+    def _ctx_body(self, k: LKernel, la: ArrayLaunch, roots: List[int], box, W, is_out) -> List[str]:
+        nodes = k.nodes
+        nd = self.ndim
+        zero = tuple(0 for _ in range(nd))
+        needed: Dict[int, Set[Tuple[int, ...]]] = {}
+        rootset = set(roots)
+        stack = [(r, zero) for r in roots]
+        # in the store phase, materialised targets are read from LDS at offset 0
+        if is_out:
+            for t in la.targets:
+                if t in la.mat:
+                    stack.append((t, zero))
+        while stack:
+            v, c = stack.pop()
+            s = needed.setdefault(v, set())
+            if c in s:
+                continue
+            s.add(c)
+            n = nodes[v]
+            if v in la.mat and v not in rootset:
+                continue
+            if n.op == "shift":
+                c2 = tuple(x - y for x, y in zip(c, n.payload))
+                stack.append((n.args[0], c2))
+            elif n.op == "arith":
+                for a_ in n.args:
+                    stack.append((a_, c))
+        lines = []
+        for v in sorted(needed):
+            n = nodes[v]
+            tn = n.ctype.c_name
+            for c in sorted(needed[v]):
+                name = "n%d_%s" % (v, _cname(c))
+                if v in la.mat and v not in rootset:
+                    b = la.lds[v].box
+                    b = tuple(b) + ((0, 0),) * (3 - len(b))
+                    UW = [self.tile[ax] + b[ax][1] - b[ax][0] for ax in range(3)]
+                    cc = tuple(c) + (0,) * (3 - nd)
+                    terms = ["l%d + %d" % (ax, box[ax][0] + cc[ax] - b[ax][0]) for ax in range(3)]
+                    idx = "(%s) * %d + (%s) * %d + (%s)" % (terms[2], UW[0] * UW[1], terms[1], UW[0],
+                                                            terms[0])
+                    lines.append("const %s %s = s%d[%s];" % (tn, name, v, idx))
+                elif n.op == "load":
+                    lines.append("const %s %s = a.ld%d[%s];" % (tn, name, n.payload,
+                                                                self._gaddr(c, is_out)))
+                elif n.op == "imm":
+                    lines.append("const %s %s = %s;" % (tn, name, c_literal(n.payload, n.ctype)))
+                elif n.op == "index":
+                    ax = n.payload
+                    lines.append("const %s %s = (%s)om_wrap(g%d + %d, a.n%d);" % (tn, name, tn, ax,
+                                                                                c[ax], ax))
+                elif n.op == "size":
+                    lines.append("const %s %s = (%s)a.n%d;" % (tn, name, tn, n.payload))
+                elif n.op == "bcast":
+                    lines.append("const %s %s = u%d;" % (tn, name, n.args[0]))
+                elif n.op == "shift":
+                    c2 = tuple(x - y for x, y in zip(c, n.payload))
+                    lines.append("const %s %s = n%d_%s;" % (tn, name, n.args[0], _cname(c2)))
+                elif n.op == "arith":
+                    op, cast_to = n.payload
+                    args = ["n%d_%s" % (a_, _cname(c)) for a_ in n.args]
+                    lines.append("const %s %s = %s;" % (tn, name, device_arith(
+                        op, args, cast_to, [nodes[a_].ctype for a_ in n.args], n.ctype)))
+                else:
+                    raise NotImplementedError(n.op)
+        return lines
+
+    def _gaddr(self, c, is_out) -> str:
+        nd = self.ndim
+        if is_out and all(x == 0 for x in c):
+            return "addr0"
+        g = []
+        for ax in range(nd):
+            g.append("om_wrap(g%d + %d, a.n%d)" % (ax, c[ax], ax))
+        g += ["0"] * (3 - nd)
+        return self._addr(g)
+
+    # ------------------------------------------------------------ machine
+    def emit(self) -> str:
+        out = ["// Generated by paraiso_amd for Orthotope Machine '%s' (gfx950)." % self.name,
+               "#include <math.h>", "#include \"om_device.h\"", "#include \"paraiso_hip.h\"", ""]
+        launchers = []
+        kdescs = []
+        for ki, sch in enumerate(self.schedules):
+            ldesc = []
+            for la in sch.launches:
+                ksrc, hsrc = self.emit_launch(ki, sch, la)
+                out.append(ksrc)
+                launchers.append(hsrc)
+                tile = list(self.tile) if la.targets else [0, 0, 0]
+                ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d}" % (
+                    ki, la.index, tile[0], tile[1], tile[2], len(la.reduces)))
+            stored = sorted(set(s for s, _ in sch.array_stores))
+            out_l = "static const pr_launch_desc k_launch_%d[] = {%s};" % (ki, ", ".join(ldesc))
+            launchers.append(out_l)
+            if stored:
+                launchers.append("static const int k_stored_%d[] = {%s};" % (
+                    ki, ", ".join(str(s) for s in stored)))
+            kdescs.append('{"%s", %d, k_launch_%d, %d, %s}' % (
+                sch.kernel.name, len(sch.launches), ki, len(stored),
+                ("k_stored_%d" % ki) if stored else "nullptr"))
+        out.extend(launchers)
+        sdesc = ['{"%s", %s, %s}' % (s.name, "PR_ARRAY" if s.dyn.realm is Realm.Array else "PR_SCALAR",
+                                     PR_CTYPE[s.dyn.ctype]) for s in self.statics]
+        out.append("static const pr_static_desc k_statics[] = {%s};" % ", ".join(sdesc))
+        out.append("static const pr_kernel_desc k_kernels[] = {%s};" % ", ".join(kdescs))
+        out.append('static const pr_machine_desc k_desc = {PR_ABI_VERSION, "%s", %d, %d, k_statics, '
+                   '%d, k_kernels, %d, %d};' % (self.name, self.ndim, self.ns, len(self.schedules),
+                                                len(self.slot_index), 1))
+        out.append('extern "C" const pr_machine_desc* pm_describe(void) { return &k_desc; }')
+        return "\n".join(out) + "\n"

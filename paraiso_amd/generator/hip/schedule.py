@@ -1,0 +1,339 @@
+"""Launch partition, LDS materialisation and halo boxes for one OM kernel.
+
+This is the MI355X replacement of the reference's subkernel split
+(``Optimization/DependencyAnalysis.hs``, ``Generator/OMTrans.hs:136-198``) and
+SASA loop (``Generator/PlanTrans.hs:406-596``).  Instead of one launch per
+write group with every Manifest array round-tripping through memory, a kernel
+becomes as few launches as global synchronisation allows:
+
+* A value's *level* is the number of Reduce steps on its dependency path.
+  Array values of level L are computed by launch L.  A Reduce over a level-L
+  array produces a level-(L+1) scalar, finished by the last workgroup of
+  launch L (the "tail"), which also evaluates the scalar arithmetic that
+  depends on it and, in the last launch, commits scalar stores.
+* Inside a launch every workgroup owns one tile of the grid.  Array values
+  needed at several stencil offsets and expensive enough to be worth it are
+  *materialised*: computed once per point over the tile grown by their halo
+  and kept in LDS.  Everything else is expanded per point and per offset in
+  registers, which is the reference's SASA scheme restricted to cheap values.
+  Loads read at any non-zero offset become LDS halo tiles.
+* Materialised values are ordered in phases separated by a workgroup barrier;
+  each phase is one cooperative loop over its box.  Boxes grow backwards from
+  the tile by the offsets at which later phases read them.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Set, Tuple
+
+from ...om.arith import Op
+from ...om.dyn import CType, Realm
+from .ir import LKernel, LNode
+
+OUT = -1                      # the store phase context
+MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
+
+
+@dataclass
+class Phase:
+    index: int
+    nodes: List[int]
+    box: Tuple[Tuple[int, int], ...]       # per axis (lo, hi) offsets around the tile
+
+
+@dataclass
+class LdsBuf:
+    node: int
+    offset: int
+    nbytes: int
+    box: Tuple[Tuple[int, int], ...]
+
+
+@dataclass
+class ArrayLaunch:
+    index: int
+    stores: List[Tuple[int, int]]          # (static idx, value id) written this launch
+    reduces: List[int]                      # reduce value ids finished in this tail
+    targets: List[int]
+    mat: Set[int]
+    phases: List[Phase]
+    lc: Dict[int, Dict[int, Set[Tuple[int, ...]]]]    # node -> context -> cursors
+    lds: Dict[int, LdsBuf]
+    lds_bytes: int
+    loads_global: Set[int]                  # loads read straight from HBM
+    tail_scalars: List[int]                 # scalar values computed in the tail
+    commit: bool                            # this tail commits the scalar stores
+
+
+@dataclass
+class KernelSchedule:
+    kernel: LKernel
+    level: List[int]
+    launches: List[ArrayLaunch]
+    slot_scalars: List[int]                 # scalar values that live in device slots
+    scalar_stores: List[Tuple[int, int]]
+    array_stores: List[Tuple[int, int]]
+    halo: int                               # largest |offset| a load is read at
+
+
+def _cost_table(k: LKernel) -> List[int]:
+    """Number of arithmetic/shift nodes in each value's full cone (shared once)."""
+    cones: List[Set[int]] = []
+    costs = []
+    for n in k.nodes:
+        s: Set[int] = set()
+        if n.op in ("arith", "shift") and n.realm is Realm.Array:
+            s.add(n.id)
+            for a in n.args:
+                s |= cones[a]
+        elif n.op == "arith":
+            pass
+        cones.append(s if len(s) < 4000 else set(list(s)[:4000]))
+        costs.append(len(s))
+    return costs
+
+
+def _add(c, d):
+    return tuple(x + y for x, y in zip(c, d))
+
+
+def _neg(v):
+    return tuple(-x for x in v)
+
+
+def schedule_kernel(k: LKernel, tile: Tuple[int, ...]) -> KernelSchedule:
+    nd = k.ndim
+    zero = tuple(0 for _ in range(nd))
+    nodes = k.nodes
+    # ---- levels
+    level = [0] * len(nodes)
+    for n in nodes:
+        if n.op in ("load", "imm", "index", "size"):
+            level[n.id] = 0
+        elif n.op == "reduce":
+            level[n.id] = level[n.args[0]] + 1
+        else:
+            level[n.id] = max([level[a] for a in n.args] + [0])
+    array_stores = [(s, v) for s, v in k.stores if nodes[v].realm is Realm.Array]
+    scalar_stores = [(s, v) for s, v in k.stores if nodes[v].realm is Realm.Scalar]
+    reduces = [n.id for n in nodes if n.op == "reduce"]
+
+    targets_by_level: Dict[int, List[int]] = {}
+    for _, v in array_stores:
+        targets_by_level.setdefault(level[v], []).append(v)
+    for r in reduces:
+        a = nodes[r].args[0]
+        targets_by_level.setdefault(level[a], []).append(a)
+    n_launch = 1 + max(list(targets_by_level) + [0])
+    # scalar values evaluated in tails: level >= 1 and used by something live
+    slot_scalars = [n.id for n in nodes if n.realm is Realm.Scalar and level[n.id] >= 1]
+
+    costs = _cost_table(k)
+    users = k.users()
+    launches = []
+    for L in range(n_launch):
+        targets = sorted(set(targets_by_level.get(L, [])))
+        stores_L = [(s, v) for s, v in array_stores if level[v] == L]
+        red_L = [r for r in reduces if level[nodes[r].args[0]] == L]
+        tail = [s for s in slot_scalars if level[s] == L + 1]
+        launches.append(_schedule_launch(k, L, targets, stores_L, red_L, tail,
+                                         L == n_launch - 1, costs, tile, zero))
+    halo = 0
+    for la in launches:
+        for ph in la.phases:
+            for lo, hi in ph.box:
+                halo = max(halo, -lo, hi)
+        for v, ctxs in la.lc.items():
+            if nodes[v].op == "load":
+                for curs in ctxs.values():
+                    for c in curs:
+                        halo = max(halo, max(abs(x) for x in c) if c else 0)
+    return KernelSchedule(k, level, launches, slot_scalars, scalar_stores, array_stores, halo)
+
+
+def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, costs,
+                     tile, zero) -> ArrayLaunch:
+    nodes = k.nodes
+    nd = k.ndim
+    # cone of array values needed by this launch
+    cone: Set[int] = set()
+    stack = list(targets)
+    while stack:
+        x = stack.pop()
+        if x in cone:
+            continue
+        cone.add(x)
+        n = nodes[x]
+        if n.op in ("arith", "shift") and n.realm is Realm.Array:
+            stack.extend(a for a in n.args)
+    loads = set(v for v in cone if nodes[v].op == "load" and nodes[v].realm is Realm.Array)
+    target_set = set(targets)
+    computed = sorted(v for v in cone if nodes[v].op in ("arith", "shift")
+                      and nodes[v].realm is Realm.Array)
+    hinted = set(v for v in computed if nodes[v].hint_manifest and v not in target_set
+                 and nodes[v].op == "arith")
+    M: Set[int] = set(loads) | hinted
+
+    def analyse(M):
+        # phases of materialised values: loads 0, others 1 + max phase they read
+        reads: Dict[int, Set[int]] = {}
+
+        def reads_of(roots):
+            seen, out, st = set(), set(), list(roots)
+            first = set(roots)
+            while st:
+                x = st.pop()
+                if x in seen:
+                    continue
+                seen.add(x)
+                if x in M and x not in first:
+                    out.add(x)
+                    continue
+                n = nodes[x]
+                if n.op in ("arith", "shift") and n.realm is Realm.Array:
+                    st.extend(n.args)
+            return out
+        phase: Dict[int, int] = {}
+        for v in sorted(M):
+            if v in loads:
+                phase[v] = 0
+        for v in sorted(M):
+            if v in loads:
+                continue
+            r = reads_of([v])
+            phase[v] = 1 + max([phase[u] for u in r] + [0])
+        out_roots = [t for t in targets if t not in M]
+        r_out = reads_of(out_roots) | set(t for t in targets if t in M)
+        out_phase = 1 + max([phase[u] for u in r_out] + [0])
+        members: Dict[int, List[int]] = {}
+        for v, p in phase.items():
+            members.setdefault(p, []).append(v)
+        # cursors per (value, phase), expanding inline values only
+        lc: Dict[int, Dict[int, Set]] = {}
+
+        def expand(P, roots):
+            rootset = set(roots)
+            st = [(r, zero) for r in roots]
+            while st:
+                x, c = st.pop()
+                d = lc.setdefault(x, {}).setdefault(P, set())
+                if c in d:
+                    continue
+                d.add(c)
+                if x in M and x not in rootset:
+                    continue
+                n = nodes[x]
+                if n.op == "shift":
+                    st.append((n.args[0], _add(c, _neg(n.payload))))
+                elif n.op == "arith" and n.realm is Realm.Array:
+                    for a in n.args:
+                        if nodes[a].realm is Realm.Array:
+                            st.append((a, c))
+        for p in sorted(members):
+            if p == 0:
+                continue
+            expand(p, sorted(members[p]))
+        expand(OUT, out_roots)
+        for t in targets:
+            if t in M:
+                lc.setdefault(t, {}).setdefault(OUT, set()).add(zero)
+        # the OUT context reads materialised targets at 0 only; drop self entries
+        for v in list(lc):
+            if v in M and phase.get(v) in lc[v]:
+                del lc[v][phase[v]]
+        return phase, out_phase, members, lc
+
+    def local_costs(M):
+        cone_m: Dict[int, Set[int]] = {}
+        lcost = {}
+        for v in computed:
+            n = nodes[v]
+            s_: Set[int] = set()
+            if n.op == "arith":
+                s_.add(v)
+            for a in n.args:
+                if a in cone_m and a not in M:
+                    s_ |= cone_m[a]
+            cone_m[v] = s_
+            lcost[v] = len(s_)
+        return lcost
+
+    for _round in range(64):
+        phase, out_phase, members, lc = analyse(M)
+        lcost = local_costs(M)
+        best, best_gain = None, 0
+        for v in computed:
+            if v in M or nodes[v].op != "arith":
+                continue
+            E = sum(len(c) for c in lc.get(v, {}).values())
+            gain = (E - 1) * lcost[v]
+            if E >= 2 and gain > best_gain:
+                best, best_gain = v, gain
+        if best is None or best_gain < MAT_GAIN:
+            break
+        M.add(best)
+    # values that ended up used once gain nothing from LDS
+    phase, out_phase, members, lc = analyse(M)
+    for v in sorted(M):
+        if v in loads:
+            uses = sum(len(c) for c in lc.get(v, {}).values())
+            nonzero = any(c != zero for cs in lc.get(v, {}).values() for c in cs)
+            if uses <= 1 and not nonzero:
+                M.discard(v)
+        else:
+            uses = sum(len(c) for c in lc.get(v, {}).values())
+            if uses <= 1:
+                M.discard(v)
+    phase, out_phase, members, lc = analyse(M)
+    # ---- boxes, from the store phase backwards
+    box: Dict[int, Tuple] = {OUT: tuple((0, 0) for _ in range(nd))}
+    phases = []
+    for p in sorted(members, reverse=True):
+        mem = sorted(members[p])
+        lo = [0] * nd
+        hi = [0] * nd
+        for v in mem:
+            for ctx, curs in lc.get(v, {}).items():
+                cb = box[ctx]
+                for c in curs:
+                    for ax in range(nd):
+                        lo[ax] = min(lo[ax], cb[ax][0] + c[ax])
+                        hi[ax] = max(hi[ax], cb[ax][1] + c[ax])
+        pb = tuple((lo[ax], hi[ax]) for ax in range(nd))
+        box[p] = pb
+        phases.append(Phase(p, mem, pb))
+    phases.sort(key=lambda ph: ph.index)
+    phases.append(Phase(out_phase, [OUT], box[OUT]))
+    # ---- LDS allocation with reuse after the last reading phase
+    last_read: Dict[int, int] = {}
+    for v in M:
+        rp = [out_phase if ctx == OUT else ctx for ctx in lc.get(v, {})]
+        last_read[v] = max(rp) if rp else phase[v]
+    lds: Dict[int, LdsBuf] = {}
+    live: List[LdsBuf] = []
+    top = 0
+    for ph in phases[:-1]:
+        live = [b for b in live if last_read[b.node] >= ph.index]
+        for v in ph.nodes:
+            vol = 1
+            for ax, (lo, hi) in enumerate(ph.box):
+                vol *= tile[ax] + hi - lo
+            nbytes = (vol * nodes[v].ctype.size + 15) // 16 * 16
+            off = _first_fit(live, nbytes)
+            b = LdsBuf(v, off, nbytes, ph.box)
+            lds[v] = b
+            live.append(b)
+            top = max(top, off + nbytes)
+    loads_global = set(v for v in loads if v not in M)
+    return ArrayLaunch(L, stores, reduces, targets, M, phases, lc, lds, top, loads_global,
+                       tail, commit)
+
+
+def _first_fit(live: List[LdsBuf], nbytes: int) -> int:
+    spans = sorted((b.offset, b.offset + b.nbytes) for b in live)
+    pos = 0
+    for a, e in spans:
+        if pos + nbytes <= a:
+            return pos
+        pos = max(pos, e)
+    return pos

@@ -1,0 +1,60 @@
+"""The machines this package builds ahead of time, one shared library each.
+
+Grid sizes are run-time parameters of a machine, so one library serves every
+size of its program.  The ratio of specific heats and the dt mode are baked
+into the OM of a Hydro program, as in the reference.
+"""
+from __future__ import annotations
+
+from fractions import Fraction
+
+from . import annotation as A
+from .generator import Language, Setup, build_machine
+from .om.dyn import CType
+
+CYC2 = (A.Condition.Cyclic,) * 2
+CYC3 = (A.Condition.Cyclic,) * 3
+
+
+def _life():
+    from .schemes.life import life_om
+    return life_om()
+
+
+def _hydro(real, gamma=Fraction(7, 5), dt_mode="cfl"):
+    def mk():
+        from .schemes.hydro import hydro_om
+        return hydro_om(real, gamma, dt_mode)
+    return mk
+
+
+def _wave3d(real=CType.Float):
+    def mk():
+        from .schemes.linear_wave import linear_wave_om
+        return linear_wave_om(3, real)
+    return mk
+
+
+# name -> (OM factory, setup, precise)
+VARIANTS = {
+    "life_i32": (_life, Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), False),
+    "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    "hydro_f64": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+}
+
+
+def build(name: str, force: bool = False) -> str:
+    mk, setup, precise = VARIANTS[name]
+    return build_machine(setup, mk(), name, precise=precise, force=force)
+
+
+def library(name: str) -> str:
+    import os
+    from .generator import GEN_DIR
+    return os.path.join(GEN_DIR, name, "lib%s.so" % name)
+
+
+def machine(name: str, local_size):
+    from .machine import Machine
+    return Machine(library(name), local_size)

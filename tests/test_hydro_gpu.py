@@ -1,0 +1,89 @@
+"""Hydro (2-D Euler, HLLC, two stages; examples/Hydro/HydroMain.hs) on the GPU
+against the CPU oracle on the same Kelvin-Helmholtz input.
+
+* precise builds (no FP contraction, correctly rounded division/sqrt) must
+  match the oracle of the same precision bit for bit;
+* the fast FP32 build must stay within 1e-5 relative L1 of the FP64 oracle.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from helpers import oracle_machine, rel_l1
+from paraiso_amd import inputs, variants
+from paraiso_amd.om.dyn import CType
+from paraiso_amd.schemes.hydro import hydro_om
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("density", "velocity0", "velocity1", "pressure")
+GAMMA = Fraction(7, 5)
+
+
+def setup_machine(m, nx, ny, seed, dtype):
+    state = inputs.kh_init(nx, ny, seed, dtype=dtype)
+    for k, v in inputs.hydro_scalars(nx, ny).items():
+        m.set(k, v)
+    for name, arr in zip(FIELDS, state):
+        m.set(name, arr)
+
+
+def run_pair(variant, real, nx, ny, steps, seed=11):
+    print("hydro case %s nx=%d ny=%d steps=%d" % (variant, nx, ny, steps), flush=True)
+    dt = np.float32 if real is CType.Float else np.float64
+    gpu = variants.machine(variant, (nx, ny))
+    setup_machine(gpu, nx, ny, seed, dt)
+    ora = oracle_machine(lambda: hydro_om(real, GAMMA), (nx, ny))
+    setup_machine(ora, nx, ny, seed, dt)
+    gpu.run("proceed", steps)
+    ora.run("proceed", steps)
+    return gpu, ora
+
+
+@pytest.mark.parametrize("variant,real", [("hydro_f64", CType.Double),
+                                          ("hydro_f32_precise", CType.Float)])
+def test_precise_bit_exact(require_gpu, variant, real):
+    gpu, ora = run_pair(variant, real, 64, 48, 20)
+    assert gpu.get("time") == ora.get("time")
+    for f in FIELDS:
+        g, o = gpu.get(f), ora.get(f)
+        assert np.array_equal(g, o), (f, float(np.abs(g - o).max()))
+
+
+def test_fast_f32_within_tolerance_of_f64(require_gpu):
+    nx, ny, steps = 96, 64, 40
+    gpu = variants.machine("hydro_f32", (nx, ny))
+    setup_machine(gpu, nx, ny, 5, np.float32)
+    ora = oracle_machine(lambda: hydro_om(CType.Double, GAMMA), (nx, ny))
+    setup_machine(ora, nx, ny, 5, np.float64)
+    gpu.run("proceed", steps)
+    ora.run("proceed", steps)
+    # velocity1 alone is a 1e-2-amplitude perturbation: its own relative L1 is
+    # dominated by FP32 rounding of O(1) quantities, so velocities are compared
+    # as a vector and the state through its conserved variables.
+    for f in ("density", "velocity0", "pressure"):
+        e = rel_l1(gpu.get(f), ora.get(f))
+        assert e < 1e-5, (f, e)
+    vg = np.stack([gpu.get("velocity0"), gpu.get("velocity1")])
+    vo = np.stack([ora.get("velocity0"), ora.get("velocity1")])
+    assert rel_l1(vg, vo) < 1e-5
+    assert rel_l1(conserved(gpu), conserved(ora)) < 1e-5
+    assert abs(gpu.get("time") - ora.get("time")) / ora.get("time") < 1e-5
+
+
+def conserved(m, gamma=1.4):
+    r, u, v, p = (m.get(f).astype(np.float64) for f in FIELDS)
+    return np.stack([r, r * u, r * v, p / (gamma - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def test_uniform_state_is_fixed_point(require_gpu):
+    m = variants.machine("hydro_f64", (32, 32))
+    for k, v in inputs.hydro_scalars(32, 32).items():
+        m.set(k, v)
+    m.set("density", 1.3)
+    m.set("velocity0", 0.2)
+    m.set("velocity1", -0.1)
+    m.set("pressure", 0.7)
+    m.run("proceed", 10)
+    assert np.allclose(m.get("density"), 1.3, rtol=0, atol=1e-14)
+    assert np.allclose(m.get("pressure"), 0.7, rtol=0, atol=1e-13)
