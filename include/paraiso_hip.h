@@ -106,6 +106,10 @@ const char* pr_last_error(const pr_machine* m);
  * same through a cached HIP graph of `count` calls. */
 int pr_run(pr_machine* m, int k, int count);
 int pr_run_graph(pr_machine* m, int k, int count);
+/* pr_run with a HIP event pair around every launch; launch_ms[l] receives the
+ * summed device time of launch l over the `count` calls (no counterpart in the
+ * reference: measurement support). */
+int pr_run_timed(pr_machine* m, int k, int count, double* launch_ms);
 
 /* ---- statics (replaces `T& name()` / `T& name(i0,i1,...)`, PlanTrans.hs:117-157) */
 int pr_static_devptr(pr_machine* m, int s, void** dev_ptr);

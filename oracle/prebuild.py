@@ -25,6 +25,7 @@ def _cases():
     out = []
     for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96)]:
         out.append((life_om, n, cyc2))
+    out.append((lambda: hydro_om(CType.Float, G), (2048, 2048), cyc2))   # bench CPU baseline
     for n in [(64, 48), (96, 64), (48, 32)]:
         out.append((lambda: hydro_om(CType.Double, G), n, cyc2))
         out.append((lambda: hydro_om(CType.Float, G), n, cyc2))

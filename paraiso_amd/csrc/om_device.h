@@ -5,10 +5,11 @@
 //                    (the reference's `(x + c + n) % n`, Generator/PlanTrans.hs:459-461,481-483),
 //   * om_max/om_min  exactly std::max/std::min (the reference's CPU Max/Min,
 //                    Generator/PlanTrans.hs:690-691): ties and NaNs resolve as there,
-//   * om_block_*     wave64 shuffle tree + LDS combine for Reduce, and the
-//                    last-workgroup finish that replaces the reference's serial
-//                    om_reduce_* (Generator/PlanTrans.hs:719) with a deterministic
-//                    fixed-order tree over per-workgroup partials.
+//   * om_block_*     wave64 shuffle tree + LDS combine for Reduce: each
+//                    workgroup writes one partial, and a one-workgroup finish
+//                    kernel folds the partials in a fixed order, replacing the
+//                    reference's serial om_reduce_* (Generator/PlanTrans.hs:719)
+//                    with a deterministic tree.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -68,19 +69,4 @@ __device__ __forceinline__ T om_partials_reduce(const T* __restrict__ part, int 
   T acc = init;
   for (int i = threadIdx.x; i < count; i += OM_NT) acc = op(acc, part[i]);
   return om_block_reduce(acc, op, scratch);
-}
-
-// Publish this workgroup's partials and learn whether it is the last one to finish.
-// The last workgroup resets the arrival counter, so the kernel can be replayed.
-__device__ __forceinline__ bool om_arrive_last(unsigned* counter, unsigned nblocks, int* flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    *flag = (prev == nblocks - 1u);
-  }
-  __syncthreads();
-  const bool last = *flag != 0;
-  if (last) __threadfence();
-  return last;
 }

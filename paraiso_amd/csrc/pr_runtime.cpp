@@ -53,6 +53,8 @@ struct pr_machine {
   std::vector<std::vector<void*>> snaps;        // snapshot buffers
   std::vector<char*> snap_scal;
   std::string err;
+  hipEvent_t* timing = nullptr;    // set while pr_run_timed records launch events
+  int timing_stride = 0, timing_call = 0;
 };
 
 namespace {
@@ -111,7 +113,11 @@ int launch_kernel_once(pr_machine* m, int k) {
     }
     ctx.ntiles = m->ntiles[k][l];
     ctx.stream = m->stream;
+    hipEvent_t* ev = m->timing ? m->timing + 2 * ((size_t)m->timing_call * m->timing_stride + l)
+                               : nullptr;
+    if (ev) (void)hipEventRecord(ev[0], m->stream);
     int rc = kd.launches[l].fn(&ctx);
+    if (ev) (void)hipEventRecord(ev[1], m->stream);
     if (rc != 0) {
       hipError_t e = hipGetLastError();
       return fail(m, PR_E_HIP, std::string("launch ") + kd.name + "[" + std::to_string(l) +
@@ -262,6 +268,34 @@ int pr_run(pr_machine* m, int k, int count) {
     if (rc) return rc;
   }
   return PR_OK;
+}
+
+int pr_run_timed(pr_machine* m, int k, int count, double* launch_ms) {
+  if (!m || k < 0 || k >= m->d->n_kernel || count < 0 || !launch_ms) return PR_E_INVALID;
+  const pr_kernel_desc& kd = m->d->kernels[k];
+  const int nl = kd.n_launch;
+  std::vector<hipEvent_t> ev((size_t)2 * nl * (count > 0 ? count : 1));
+  for (auto& e : ev) PR_HIP(m, hipEventCreate(&e));
+  m->timing = ev.data();
+  m->timing_stride = nl;
+  int rc = PR_OK;
+  for (int c = 0; c < count && rc == PR_OK; ++c) {
+    m->timing_call = c;
+    rc = launch_kernel_once(m, k);
+  }
+  m->timing = nullptr;
+  if (rc == PR_OK) {
+    PR_HIP(m, hipStreamSynchronize(m->stream));
+    for (int l = 0; l < nl; ++l) launch_ms[l] = 0.0;
+    for (int c = 0; c < count; ++c)
+      for (int l = 0; l < nl; ++l) {
+        float ms = 0.f;
+        PR_HIP(m, hipEventElapsedTime(&ms, ev[2 * ((size_t)c * nl + l)], ev[2 * ((size_t)c * nl + l) + 1]));
+        launch_ms[l] += ms;
+      }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int pr_run_graph(pr_machine* m, int k, int count) {

@@ -14,10 +14,12 @@ from the hand-written templates in ``csrc/om_device.h``:
 4. the store phase over the tile: stores go to the ping-pong "next" buffers,
    Reduce arguments fold into per-thread accumulators (FP64 for floating
    sums),
-5. the reduction epilogue: wave64 shuffle tree, per-workgroup partials, and
-   the last workgroup to arrive finishes every Reduce in a fixed order, runs
-   the scalar tail (``om_reduce_*``/``om_broadcast``, ``PlanTrans.hs:566-569``)
-   and, in the kernel's last launch, commits the scalar stores.
+5. the reduction epilogue: wave64 shuffle tree and one partial per workgroup;
+   a one-workgroup ``_fin`` kernel, ordered after it on the stream, folds the
+   partials in a fixed order, runs the scalar tail (``om_reduce_*`` /
+   ``om_broadcast``, ``PlanTrans.hs:566-569``) and, in the kernel's last launch,
+   commits the scalar stores.  Stream order replaces a device-wide fence per
+   workgroup, which on a multi-XCD part would write back L2 thousands of times.
 
 Per-point expressions follow the reference's SASA naming (``a<node>_<cursor>``,
 ``PlanTrans.hs:625-644``) as ``n<value>_<cursor>``.
@@ -174,7 +176,6 @@ class MachineEmitter:
         w("extern \"C\" __global__ void __launch_bounds__(%d) %s(Args_%s a) {" % (NT, fn, fn))
         if la.lds_bytes:
             w("  __shared__ __align__(16) char smem[%d];" % la.lds_bytes)
-        w("  __shared__ int om_flag;")
         acc_types = set()
         for r in la.reduces:
             acc_types.add(acc_type(nodes[r].payload, nodes[nodes[r].args[0]].ctype))
@@ -209,45 +210,51 @@ class MachineEmitter:
         if has_work:
             for ph in la.phases:
                 self._emit_phase(k, la, ph, w)
-        # ---- epilogue
+        # ---- epilogue: per-workgroup partials; the finish kernel does the rest
         needs_tail = bool(la.reduces) or bool(la.tail_scalars) or la.commit
-        if needs_tail:
-            if has_work:
-                for r in la.reduces:
-                    a_ct = nodes[nodes[r].args[0]].ctype
-                    at = acc_type(nodes[r].payload, a_ct)
-                    j = la.reduces.index(r)
-                    w("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
-                        at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
-                    w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + blockIdx.x] = br; }" % (
-                        at.c_name, j))
-                w("  if (om_arrive_last(a.counter, a.ntiles, &om_flag)) {")
-            else:
-                w("  {")
+        if has_work:
             for j, r in enumerate(la.reduces):
                 a_ct = nodes[nodes[r].args[0]].ctype
                 at = acc_type(nodes[r].payload, a_ct)
-                src = "acc%d" % r if not has_work else "om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.ntiles, (%s)%s, %s, om_scr_%s)" % (
-                    at.c_name, j, at.c_name, acc_init(nodes[r].payload, at),
-                    acc_op(nodes[r].payload, at), at.c_name)
-                w("    const %s fr%d = %s;" % (at.c_name, r, src))
-            w("    if (threadIdx.x == 0) {")
+                w("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
+                    at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
+                w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + blockIdx.x] = br; }" % (
+                    at.c_name, j))
+        w("}")
+        fin = []
+        if needs_tail:
+            f = fin.append
+            f("extern \"C\" __global__ void __launch_bounds__(%d) %s_fin(Args_%s a) {" % (NT, fn, fn))
+            for at in sorted(acc_types, key=lambda c: c.value):
+                f("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
+            for n in nodes:
+                if n.realm is not Realm.Scalar:
+                    continue
+                lvl = sch.level[n.id]
+                if lvl == 0:
+                    f("  const %s u%d = %s;" % (n.ctype.c_name, n.id, self.scalar_expr(k, n.id)))
+                elif lvl <= L:
+                    f("  const %s u%d = *(const %s*)(a.scal + %d);" % (
+                        n.ctype.c_name, n.id, n.ctype.c_name, self.slot_off(ki, n.id)))
+            for j, r in enumerate(la.reduces):
+                a_ct = nodes[nodes[r].args[0]].ctype
+                at = acc_type(nodes[r].payload, a_ct)
+                f("  const %s fr%d = om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.ntiles, "
+                  "(%s)%s, %s, om_scr_%s);" % (at.c_name, r, at.c_name, j, at.c_name,
+                                              acc_init(nodes[r].payload, at),
+                                              acc_op(nodes[r].payload, at), at.c_name))
+            f("  if (threadIdx.x == 0) {")
             for v in la.tail_scalars:
                 n = nodes[v]
-                if n.op == "reduce":
-                    expr = "(%s)(fr%d)" % (n.ctype.c_name, v)
-                else:
-                    expr = self.scalar_expr(k, v)
-                w("      const %s u%d = %s;" % (n.ctype.c_name, v, expr))
-                w("      *(%s*)(a.scal + %d) = u%d;" % (n.ctype.c_name, self.slot_off(ki, v), v))
+                expr = "(%s)(fr%d)" % (n.ctype.c_name, v) if n.op == "reduce" else self.scalar_expr(k, v)
+                f("    const %s u%d = %s;" % (n.ctype.c_name, v, expr))
+                f("    *(%s*)(a.scal + %d) = u%d;" % (n.ctype.c_name, self.slot_off(ki, v), v))
             if la.commit:
-                for s, v in sch.scalar_stores:
-                    w("      *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s, v))
-            if has_work:
-                w("      *a.counter = 0u;")
-            w("    }")
-            w("  }")
-        w("}")
+                for s_, v in sch.scalar_stores:
+                    f("    *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s_, v))
+            f("  }")
+            f("}")
+        body.extend(fin)
         kernel_src = struct + "\n".join(body) + "\n"
 
         # ---- host launcher
@@ -261,8 +268,12 @@ class MachineEmitter:
         h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
         h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
         h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
-        h.append("  hipLaunchKernelGGL(%s, dim3(c->ntiles), dim3(%d), 0, (hipStream_t)c->stream, a);"
-                 % (fn, NT))
+        if has_work:
+            h.append("  hipLaunchKernelGGL(%s, dim3(c->ntiles), dim3(%d), 0, (hipStream_t)c->stream, a);"
+                     % (fn, NT))
+        if needs_tail:
+            h.append("  hipLaunchKernelGGL(%s_fin, dim3(1), dim3(%d), 0, (hipStream_t)c->stream, a);"
+                     % (fn, NT))
         h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"

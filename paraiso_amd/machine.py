@@ -44,7 +44,7 @@ class PrMachineDesc(ctypes.Structure):
 
 
 RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_last_error",
-                   "pr_run", "pr_run_graph", "pr_static_devptr", "pr_static_write",
+                   "pr_run", "pr_run_graph", "pr_run_timed", "pr_static_devptr", "pr_static_write",
                    "pr_static_read", "pr_static_bytes", "pr_size", "pr_cells",
                    "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load"]
 
@@ -68,6 +68,7 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_last_error.restype = ctypes.c_char_p
     lib.pr_run.argtypes = [vp, i, i]
     lib.pr_run_graph.argtypes = [vp, i, i]
+    lib.pr_run_timed.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_double)]
     lib.pr_static_devptr.argtypes = [vp, i, ctypes.POINTER(vp)]
     lib.pr_static_write.argtypes = [vp, i, vp, sz, i]
     lib.pr_static_read.argtypes = [vp, i, vp, sz, i]
@@ -145,6 +146,14 @@ class Machine:
         k = self.kernel_index(kernel)
         fn = self.rt.pr_run_graph if graph else self.rt.pr_run
         self._check(fn(self.h, k, count), "run %s" % kernel)
+
+    def run_timed(self, kernel, count: int = 1):
+        """Run with an event pair around every launch; returns ms per launch index (summed)."""
+        k = self.kernel_index(kernel)
+        nl = self.desc_ptr.contents.kernels[k].n_launch
+        out = (ctypes.c_double * nl)()
+        self._check(self.rt.pr_run_timed(self.h, k, count, out), "run_timed %s" % kernel)
+        return list(out)
 
     def sync(self):
         self._check(self.rt.pr_sync(self.h), "sync")
