@@ -59,10 +59,13 @@ typedef struct pr_launch_ctx {
 
 typedef int (*pr_launch_fn)(const pr_launch_ctx* ctx);
 
+enum pr_static_flags { PR_STATIC_SCRATCH = 1 };
+
 typedef struct pr_static_desc {
   const char* name;
   int realm;               /* pr_realm */
   int ctype;               /* pr_ctype */
+  int flags;               /* PR_STATIC_SCRATCH: internal array between two launches of a kernel */
 } pr_static_desc;
 
 typedef struct pr_launch_desc {
@@ -121,11 +124,21 @@ size_t pr_static_bytes(const pr_machine* m, int s);
 int pr_size(const pr_machine* m, int axis);
 long long pr_cells(const pr_machine* m);
 
-/* ---- state snapshots for checkpointed adjoints (no counterpart in the reference)
- * A snapshot holds copies of every array static and the scalar statics. */
-int pr_snapshot_alloc(pr_machine* m, int n_slots);
+/* ---- checkpointed adjoints (no counterpart in the reference: the reference
+ * has no adjoint).  A snapshot holds copies of the listed statics.
+ * pr_adjoint_forward runs `nsteps` calls of kernel `fwd`, saving a checkpoint
+ * every `every` calls into slots [0, ceil(nsteps/every)).
+ * pr_adjoint_backward then walks the segments backwards: restore the
+ * segment's checkpoint, replay it storing each state in slots
+ * [S, S+every), and run kernel `adj` once per state in reverse order.  With
+ * the adjoint statics holding dJ/du_N on entry they hold dJ/du_0 on return,
+ * and the snapshotted statics hold u_0.  Needs ceil(nsteps/every) + every
+ * slots. */
+int pr_snapshot_alloc(pr_machine* m, int n_slots, const int* statics, int n_statics);
 int pr_snapshot_save(pr_machine* m, int slot);
 int pr_snapshot_load(pr_machine* m, int slot);
+int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every);
+int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,7 @@ struct pr_machine {
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::vector<std::vector<void*>> snaps;        // snapshot buffers
   std::vector<char*> snap_scal;
+  std::vector<int> snap_sel;                    // statics a snapshot holds
   std::string err;
   hipEvent_t* timing = nullptr;    // set while pr_run_timed records launch events
   int timing_stride = 0, timing_call = 0;
@@ -370,12 +371,26 @@ int pr_size(const pr_machine* m, int axis) {
 
 long long pr_cells(const pr_machine* m) { return m ? m->cells : 0; }
 
-int pr_snapshot_alloc(pr_machine* m, int n_slots) {
-  if (!m || n_slots < 0) return PR_E_INVALID;
+int pr_snapshot_alloc(pr_machine* m, int n_slots, const int* statics, int n_statics) {
+  if (!m || n_slots < 0 || n_statics < 0 || (n_statics > 0 && !statics)) return PR_E_INVALID;
   const int ns = m->d->n_static;
+  std::vector<int> sel(statics, statics + n_statics);
+  for (int s : sel)
+    if (s < 0 || s >= ns) return fail(m, PR_E_INVALID, "snapshot static index out of range");
+  if (!m->snaps.empty() && sel != m->snap_sel) {
+    // a different selection: drop the old snapshots
+    for (auto& v : m->snaps)
+      for (void* p : v)
+        if (p) (void)hipFree(p);
+    for (char* p : m->snap_scal)
+      if (p) (void)hipFree(p);
+    m->snaps.clear();
+    m->snap_scal.clear();
+  }
+  m->snap_sel = sel;
   while ((int)m->snaps.size() < n_slots) {
     std::vector<void*> v(ns, nullptr);
-    for (int s = 0; s < ns; ++s) {
+    for (int s : sel) {
       if (m->d->statics[s].realm != PR_ARRAY) continue;
       if (hipMalloc(&v[s], pr_static_bytes(m, s)) != hipSuccess) {
         for (void* p : v)
@@ -384,7 +399,7 @@ int pr_snapshot_alloc(pr_machine* m, int n_slots) {
       }
     }
     char* sc = nullptr;
-    if (hipMalloc(&sc, 8 * ns) != hipSuccess) return fail(m, PR_E_NOMEM, "snapshot scalars");
+    if (hipMalloc(&sc, 8 * (size_t)ns) != hipSuccess) return fail(m, PR_E_NOMEM, "snapshot scalars");
     m->snaps.push_back(v);
     m->snap_scal.push_back(sc);
   }
@@ -393,25 +408,62 @@ int pr_snapshot_alloc(pr_machine* m, int n_slots) {
 
 int pr_snapshot_save(pr_machine* m, int slot) {
   if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
-  const int ns = m->d->n_static;
-  for (int s = 0; s < ns; ++s) {
-    if (m->d->statics[s].realm != PR_ARRAY) continue;
-    PR_HIP(m, hipMemcpyAsync(m->snaps[slot][s], m->buf[m->cur[s]][s], pr_static_bytes(m, s),
-                             hipMemcpyDeviceToDevice, m->stream));
+  for (int s : m->snap_sel) {
+    if (m->d->statics[s].realm == PR_ARRAY)
+      PR_HIP(m, hipMemcpyAsync(m->snaps[slot][s], m->buf[m->cur[s]][s], pr_static_bytes(m, s),
+                               hipMemcpyDeviceToDevice, m->stream));
+    else
+      PR_HIP(m, hipMemcpyAsync(m->snap_scal[slot] + 8 * s, m->scal + 8 * s, 8,
+                               hipMemcpyDeviceToDevice, m->stream));
   }
-  PR_HIP(m, hipMemcpyAsync(m->snap_scal[slot], m->scal, 8 * ns, hipMemcpyDeviceToDevice, m->stream));
   return PR_OK;
 }
 
 int pr_snapshot_load(pr_machine* m, int slot) {
   if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
-  const int ns = m->d->n_static;
-  for (int s = 0; s < ns; ++s) {
-    if (m->d->statics[s].realm != PR_ARRAY) continue;
-    PR_HIP(m, hipMemcpyAsync(m->buf[m->cur[s]][s], m->snaps[slot][s], pr_static_bytes(m, s),
-                             hipMemcpyDeviceToDevice, m->stream));
+  for (int s : m->snap_sel) {
+    if (m->d->statics[s].realm == PR_ARRAY)
+      PR_HIP(m, hipMemcpyAsync(m->buf[m->cur[s]][s], m->snaps[slot][s], pr_static_bytes(m, s),
+                               hipMemcpyDeviceToDevice, m->stream));
+    else
+      PR_HIP(m, hipMemcpyAsync(m->scal + 8 * s, m->snap_scal[slot] + 8 * s, 8,
+                               hipMemcpyDeviceToDevice, m->stream));
   }
-  PR_HIP(m, hipMemcpyAsync(m->scal, m->snap_scal[slot], 8 * ns, hipMemcpyDeviceToDevice, m->stream));
+  return PR_OK;
+}
+
+int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {
+  if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
+  const int S = (nsteps + every - 1) / every;
+  if ((int)m->snaps.size() < S + every)
+    return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
+  for (int n = 0; n < nsteps; ++n) {
+    int rc;
+    if (n % every == 0 && (rc = pr_snapshot_save(m, n / every))) return rc;
+    if ((rc = launch_kernel_once(m, fwd))) return rc;
+  }
+  return PR_OK;
+}
+
+int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every) {
+  if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
+  const int S = (nsteps + every - 1) / every;
+  if ((int)m->snaps.size() < S + every)
+    return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
+  int rc;
+  for (int seg = S - 1; seg >= 0; --seg) {
+    const int start = seg * every;
+    const int len = (nsteps - start < every) ? nsteps - start : every;
+    if ((rc = pr_snapshot_load(m, seg))) return rc;
+    for (int j = 0; j < len; ++j) {
+      if ((rc = pr_snapshot_save(m, S + j))) return rc;
+      if (j + 1 < len && (rc = launch_kernel_once(m, fwd))) return rc;
+    }
+    for (int j = len - 1; j >= 0; --j) {
+      if ((rc = pr_snapshot_load(m, S + j))) return rc;
+      if ((rc = launch_kernel_once(m, adj))) return rc;
+    }
+  }
   return PR_OK;
 }
 

@@ -125,6 +125,18 @@ class MachineEmitter:
         self.schedules = schedules
         self.tile = tuple(tile) + (1,) * (3 - len(tile))
         self.ns = len(statics)
+        self.n_user = len(statics)
+        # scratch arrays of cut launches become internal Array statics after the user's
+        from ...om.dyn import DynValue
+        from ...om.graph import Named
+        from .schedule import SCRATCH_BASE
+        self.statics = list(statics)
+        self.scratch_map: Dict[Tuple[int, int], int] = {}
+        for ki, sch in enumerate(schedules):
+            for j, (v, ct) in enumerate(sch.scratch):
+                self.scratch_map[(ki, SCRATCH_BASE + j)] = len(self.statics)
+                self.statics.append(Named("__scratch_k%d_%d" % (ki, j), DynValue(Realm.Array, ct)))
+        self.ns = len(self.statics)
         # slot numbering across kernels
         self.slot_index: Dict[Tuple[int, int], int] = {}
         for ki, sch in enumerate(schedules):
@@ -132,6 +144,12 @@ class MachineEmitter:
                 self.slot_index[(ki, v)] = len(self.slot_index)
 
     # ------------------------------------------------------------ helpers
+    def sidx(self, ki, s) -> int:
+        return self.scratch_map.get((ki, s), s)
+
+    def is_scratch(self, s) -> bool:
+        return s >= self.n_user
+
     def slot_off(self, ki, v) -> int:
         return 8 * (self.ns + self.slot_index[(ki, v)])
 
@@ -151,17 +169,18 @@ class MachineEmitter:
 
     # ------------------------------------------------------------ per launch
     def emit_launch(self, ki: int, sch: KernelSchedule, la: ArrayLaunch) -> Tuple[str, str]:
-        k = sch.kernel
+        k = la.kernel_override or sch.kernel
         nodes = k.nodes
         nd = self.ndim
         T = self.tile
         fn = "om_k%d_l%d" % (ki, la.index)
+        self._ki = ki
         has_work = bool(la.targets)
         L = la.index
         # ---- argument struct
-        loads = sorted(set(nodes[v].payload for v in la.lc if nodes[v].op == "load"
+        loads = sorted(set(self.sidx(ki, nodes[v].payload) for v in la.lc if nodes[v].op == "load"
                            and nodes[v].realm is Realm.Array))
-        stores = sorted(set(s for s, _ in la.stores))
+        stores = sorted(set(self.sidx(ki, s) for s, _ in la.stores))
         fields = []
         for s in loads:
             fields.append("const %s* __restrict__ ld%d;" % (self.statics[s].dyn.ctype.c_name, s))
@@ -264,7 +283,8 @@ class MachineEmitter:
         for s in loads:
             h.append("  a.ld%d = (const %s*)c->cur[%d];" % (s, self.statics[s].dyn.ctype.c_name, s))
         for s in stores:
-            h.append("  a.st%d = (%s*)c->next[%d];" % (s, self.statics[s].dyn.ctype.c_name, s))
+            src = "cur" if self.is_scratch(s) else "next"
+            h.append("  a.st%d = (%s*)c->%s[%d];" % (s, self.statics[s].dyn.ctype.c_name, src, s))
         h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
         h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
         h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
@@ -308,7 +328,7 @@ class MachineEmitter:
         zero = tuple(0 for _ in range(nd))
         if is_out:
             for s, v in la.stores:
-                w("    a.st%d[addr0] = %s;" % (s, self._val_name(k, la, v, zero, roots)))
+                w("    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s), self._val_name(k, la, v, zero, roots)))
             for r in la.reduces:
                 arg = nodes[r].args[0]
                 at = acc_type(nodes[r].payload, nodes[arg].ctype)
@@ -373,7 +393,7 @@ class MachineEmitter:
                                                             terms[0])
                     lines.append("const %s %s = s%d[%s];" % (tn, name, v, idx))
                 elif n.op == "load":
-                    lines.append("const %s %s = a.ld%d[%s];" % (tn, name, n.payload,
+                    lines.append("const %s %s = a.ld%d[%s];" % (tn, name, self.sidx(self._ki, n.payload),
                                                                 self._gaddr(c, is_out)))
                 elif n.op == "imm":
                     lines.append("const %s %s = %s;" % (tn, name, c_literal(n.payload, n.ctype)))
@@ -432,8 +452,9 @@ class MachineEmitter:
                 sch.kernel.name, len(sch.launches), ki, len(stored),
                 ("k_stored_%d" % ki) if stored else "nullptr"))
         out.extend(launchers)
-        sdesc = ['{"%s", %s, %s}' % (s.name, "PR_ARRAY" if s.dyn.realm is Realm.Array else "PR_SCALAR",
-                                     PR_CTYPE[s.dyn.ctype]) for s in self.statics]
+        sdesc = ['{"%s", %s, %s, %d}' % (s.name, "PR_ARRAY" if s.dyn.realm is Realm.Array else "PR_SCALAR",
+                                         PR_CTYPE[s.dyn.ctype], 1 if i >= self.n_user else 0)
+                 for i, s in enumerate(self.statics)]
         out.append("static const pr_static_desc k_statics[] = {%s};" % ", ".join(sdesc))
         out.append("static const pr_kernel_desc k_kernels[] = {%s};" % ", ".join(kdescs))
         out.append('static const pr_machine_desc k_desc = {PR_ABI_VERSION, "%s", %d, %d, k_statics, '

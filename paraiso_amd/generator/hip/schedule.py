@@ -32,6 +32,7 @@ from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
+LDS_BUDGET = 64 * 1024        # bytes per workgroup; larger launches are cut (two workgroups per CU)
 
 
 @dataclass
@@ -63,6 +64,7 @@ class ArrayLaunch:
     loads_global: Set[int]                  # loads read straight from HBM
     tail_scalars: List[int]                 # scalar values computed in the tail
     commit: bool                            # this tail commits the scalar stores
+    kernel_override: Optional[LKernel] = None   # node list with scratch reads (cut launches)
 
 
 @dataclass
@@ -74,6 +76,7 @@ class KernelSchedule:
     scalar_stores: List[Tuple[int, int]]
     array_stores: List[Tuple[int, int]]
     halo: int                               # largest |offset| a load is read at
+    scratch: List[Tuple[int, CType]] = field(default_factory=list)   # (value id, type) per scratch array
 
 
 def _cost_table(k: LKernel) -> List[int]:
@@ -128,16 +131,18 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...]) -> KernelSchedule:
     # scalar values evaluated in tails: level >= 1 and used by something live
     slot_scalars = [n.id for n in nodes if n.realm is Realm.Scalar and level[n.id] >= 1]
 
-    costs = _cost_table(k)
-    users = k.users()
+    costs = None
     launches = []
+    scratch: List[Tuple[int, CType]] = []
     for L in range(n_launch):
         targets = sorted(set(targets_by_level.get(L, [])))
         stores_L = [(s, v) for s, v in array_stores if level[v] == L]
         red_L = [r for r in reduces if level[nodes[r].args[0]] == L]
         tail = [s for s in slot_scalars if level[s] == L + 1]
-        launches.append(_schedule_launch(k, L, targets, stores_L, red_L, tail,
-                                         L == n_launch - 1, costs, tile, zero))
+        launches.extend(_schedule_level(k, L, targets, stores_L, red_L, tail,
+                                        L == n_launch - 1, costs, tile, zero, scratch))
+    for i, la in enumerate(launches):
+        la.index = i
     halo = 0
     for la in launches:
         for ph in la.phases:
@@ -148,7 +153,63 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...]) -> KernelSchedule:
                 for curs in ctxs.values():
                     for c in curs:
                         halo = max(halo, max(abs(x) for x in c) if c else 0)
-    return KernelSchedule(k, level, launches, slot_scalars, scalar_stores, array_stores, halo)
+    return KernelSchedule(k, level, launches, slot_scalars, scalar_stores, array_stores, halo,
+                          scratch)
+
+
+SCRATCH_BASE = 1 << 20        # scratch arrays are numbered from here until the emitter places them
+
+
+def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, scratch,
+                    depth=0) -> List[ArrayLaunch]:
+    """Schedule one level; cut it into consecutive launches while LDS is over budget.
+
+    A cut after phase q stores every materialised value computed at or before
+    q and read after it into a scratch array (its exact bits); the second
+    launch reads those arrays as halo tiles, like loads of statics."""
+    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero)
+    inner = [ph for ph in la.phases[:-1] if ph.index > 0]
+    if la.lds_bytes <= LDS_BUDGET or len(inner) < 2 or depth > 6:
+        return [la]
+    nodes = k.nodes
+    # cut where half of the computed materialised values are behind us
+    n_mat = sum(len(ph.nodes) for ph in inner)
+    acc, q = 0, inner[0].index
+    for ph in inner[:-1]:
+        acc += len(ph.nodes)
+        q = ph.index
+        if acc * 2 >= n_mat:
+            break
+    phase_of = {v: ph.index for ph in la.phases[:-1] for v in ph.nodes}
+    out_phase = la.phases[-1].index
+    cut = []
+    for ph in inner:
+        if ph.index > q:
+            continue
+        for v in ph.nodes:
+            readers = [out_phase if ctx == OUT else ctx for ctx in la.lc.get(v, {})]
+            if any(r > q for r in readers):
+                cut.append(v)
+    if not cut:
+        return [la]
+    # first launch: compute the cut values and store them into scratch arrays
+    first_stores = []
+    for v in cut:
+        sidx = SCRATCH_BASE + len(scratch)
+        scratch.append((v, nodes[v].ctype))
+        first_stores.append((sidx, v))
+    first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, costs, tile, zero,
+                            scratch, depth + 1)
+    # second launch: the same targets, with the cut values read back as loads
+    sidx_of = {v: s for s, v in first_stores}
+    nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of else n
+              for n in nodes]
+    k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
+    second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, costs, tile, zero,
+                             scratch, depth + 1)
+    for la2 in second:
+        la2.kernel_override = k2
+    return first + second
 
 
 def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, costs,
@@ -258,20 +319,71 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             lcost[v] = len(s_)
         return lcost
 
+    def layout(M):
+        phase, out_phase, members, lc = analyse(M)
+        box: Dict[int, Tuple] = {OUT: tuple((0, 0) for _ in range(nd))}
+        phases = []
+        for p in sorted(members, reverse=True):
+            mem = sorted(members[p])
+            lo = [0] * nd
+            hi = [0] * nd
+            for v in mem:
+                for ctx, curs in lc.get(v, {}).items():
+                    cb = box[ctx]
+                    for c in curs:
+                        for ax in range(nd):
+                            lo[ax] = min(lo[ax], cb[ax][0] + c[ax])
+                            hi[ax] = max(hi[ax], cb[ax][1] + c[ax])
+            pb = tuple((lo[ax], hi[ax]) for ax in range(nd))
+            box[p] = pb
+            phases.append(Phase(p, mem, pb))
+        phases.sort(key=lambda ph: ph.index)
+        phases.append(Phase(out_phase, [OUT], box[OUT]))
+        last_read: Dict[int, int] = {}
+        for v in M:
+            rp = [out_phase if ctx == OUT else ctx for ctx in lc.get(v, {})]
+            last_read[v] = max(rp) if rp else phase[v]
+        lds: Dict[int, LdsBuf] = {}
+        live: List[LdsBuf] = []
+        top = 0
+        for ph in phases[:-1]:
+            live = [b for b in live if last_read[b.node] >= ph.index]
+            for v in ph.nodes:
+                vol = 1
+                for ax, (lo, hi) in enumerate(ph.box):
+                    vol *= tile[ax] + hi - lo
+                nbytes = (vol * nodes[v].ctype.size + 15) // 16 * 16
+                off = _first_fit(live, nbytes)
+                b = LdsBuf(v, off, nbytes, ph.box)
+                lds[v] = b
+                live.append(b)
+                top = max(top, off + nbytes)
+        return phase, out_phase, members, lc, phases, lds, top
+
     for _round in range(64):
         phase, out_phase, members, lc = analyse(M)
         lcost = local_costs(M)
-        best, best_gain = None, 0
+        ranked = []
         for v in computed:
             if v in M or nodes[v].op != "arith":
                 continue
             E = sum(len(c) for c in lc.get(v, {}).values())
             gain = (E - 1) * lcost[v]
-            if E >= 2 and gain > best_gain:
-                best, best_gain = v, gain
-        if best is None or best_gain < MAT_GAIN:
+            if E >= 2 and gain >= MAT_GAIN:
+                ranked.append((gain, v))
+        if not ranked:
             break
-        M.add(best)
+        ranked.sort(reverse=True)
+        added = False
+        for gain, v in ranked[:4]:
+            trial = set(M) | {v}
+            if layout(trial)[6] <= LDS_BUDGET or layout(M)[6] > LDS_BUDGET:
+                if layout(trial)[6] <= max(LDS_BUDGET, layout(M)[6]):
+                    M = trial
+                    added = True
+                    break
+        if not added:
+            break
     # values that ended up used once gain nothing from LDS
     phase, out_phase, members, lc = analyse(M)
     for v in sorted(M):
@@ -284,46 +396,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             uses = sum(len(c) for c in lc.get(v, {}).values())
             if uses <= 1:
                 M.discard(v)
-    phase, out_phase, members, lc = analyse(M)
-    # ---- boxes, from the store phase backwards
-    box: Dict[int, Tuple] = {OUT: tuple((0, 0) for _ in range(nd))}
-    phases = []
-    for p in sorted(members, reverse=True):
-        mem = sorted(members[p])
-        lo = [0] * nd
-        hi = [0] * nd
-        for v in mem:
-            for ctx, curs in lc.get(v, {}).items():
-                cb = box[ctx]
-                for c in curs:
-                    for ax in range(nd):
-                        lo[ax] = min(lo[ax], cb[ax][0] + c[ax])
-                        hi[ax] = max(hi[ax], cb[ax][1] + c[ax])
-        pb = tuple((lo[ax], hi[ax]) for ax in range(nd))
-        box[p] = pb
-        phases.append(Phase(p, mem, pb))
-    phases.sort(key=lambda ph: ph.index)
-    phases.append(Phase(out_phase, [OUT], box[OUT]))
-    # ---- LDS allocation with reuse after the last reading phase
-    last_read: Dict[int, int] = {}
-    for v in M:
-        rp = [out_phase if ctx == OUT else ctx for ctx in lc.get(v, {})]
-        last_read[v] = max(rp) if rp else phase[v]
-    lds: Dict[int, LdsBuf] = {}
-    live: List[LdsBuf] = []
-    top = 0
-    for ph in phases[:-1]:
-        live = [b for b in live if last_read[b.node] >= ph.index]
-        for v in ph.nodes:
-            vol = 1
-            for ax, (lo, hi) in enumerate(ph.box):
-                vol *= tile[ax] + hi - lo
-            nbytes = (vol * nodes[v].ctype.size + 15) // 16 * 16
-            off = _first_fit(live, nbytes)
-            b = LdsBuf(v, off, nbytes, ph.box)
-            lds[v] = b
-            live.append(b)
-            top = max(top, off + nbytes)
+    phase, out_phase, members, lc, phases, lds, top = layout(M)
     loads_global = set(v for v in loads if v not in M)
     return ArrayLaunch(L, stores, reduces, targets, M, phases, lc, lds, top, loads_global,
                        tail, commit)

@@ -23,7 +23,8 @@ PR_CT = {0: CType.Bool, 1: CType.Int, 2: CType.Float, 3: CType.Double}
 
 
 class PrStaticDesc(ctypes.Structure):
-    _fields_ = [("name", ctypes.c_char_p), ("realm", ctypes.c_int), ("ctype", ctypes.c_int)]
+    _fields_ = [("name", ctypes.c_char_p), ("realm", ctypes.c_int), ("ctype", ctypes.c_int),
+                ("flags", ctypes.c_int)]
 
 
 class PrLaunchDesc(ctypes.Structure):
@@ -46,7 +47,8 @@ class PrMachineDesc(ctypes.Structure):
 RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_last_error",
                    "pr_run", "pr_run_graph", "pr_run_timed", "pr_static_devptr", "pr_static_write",
                    "pr_static_read", "pr_static_bytes", "pr_size", "pr_cells",
-                   "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load"]
+                   "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load",
+                   "pr_adjoint_forward", "pr_adjoint_backward"]
 
 _RT = None
 
@@ -77,7 +79,9 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_size.argtypes = [vp, i]
     lib.pr_cells.argtypes = [vp]
     lib.pr_cells.restype = ctypes.c_longlong
-    lib.pr_snapshot_alloc.argtypes = [vp, i]
+    lib.pr_snapshot_alloc.argtypes = [vp, i, ctypes.POINTER(i), i]
+    lib.pr_adjoint_forward.argtypes = [vp, i, i, i]
+    lib.pr_adjoint_backward.argtypes = [vp, i, i, i, i]
     lib.pr_snapshot_save.argtypes = [vp, i]
     lib.pr_snapshot_load.argtypes = [vp, i]
     if path is None:
@@ -204,11 +208,22 @@ class Machine:
         return int(self.rt.pr_static_bytes(self.h, self.static_index(name)))
 
     # ---------------------------------------------------------------- snapshots
-    def snapshot_alloc(self, n: int):
-        self._check(self.rt.pr_snapshot_alloc(self.h, n), "snapshot_alloc")
+    def snapshot_alloc(self, n: int, statics):
+        idx = [self.static_index(x) for x in statics]
+        arr = (ctypes.c_int * max(1, len(idx)))(*idx)
+        self._check(self.rt.pr_snapshot_alloc(self.h, n, arr, len(idx)), "snapshot_alloc")
 
     def snapshot_save(self, slot: int):
         self._check(self.rt.pr_snapshot_save(self.h, slot), "snapshot_save")
 
     def snapshot_load(self, slot: int):
         self._check(self.rt.pr_snapshot_load(self.h, slot), "snapshot_load")
+
+    def adjoint_forward(self, fwd, nsteps: int, every: int):
+        self._check(self.rt.pr_adjoint_forward(self.h, self.kernel_index(fwd), nsteps, every),
+                    "adjoint_forward")
+
+    def adjoint_backward(self, fwd, adj, nsteps: int, every: int):
+        self._check(self.rt.pr_adjoint_backward(self.h, self.kernel_index(fwd),
+                                                self.kernel_index(adj), nsteps, every),
+                    "adjoint_backward")

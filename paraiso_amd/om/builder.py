@@ -58,7 +58,16 @@ class Value:
 
 
 def _lit(x):
-    """Normalise a Python literal; floats become exact decimal fractions."""
+    """Normalise a Python literal; floats become exact decimal fractions.
+
+    numpy scalars (np.float64 subclasses float) keep their exact binary value."""
+    import numpy as _np
+    if isinstance(x, _np.generic):
+        if isinstance(x, _np.bool_):
+            return bool(x)
+        if isinstance(x, _np.integer):
+            return int(x)
+        return Fraction(float(x))
     if isinstance(x, bool):
         return x
     if isinstance(x, int):

@@ -28,10 +28,29 @@ def _hydro(real, gamma=Fraction(7, 5), dt_mode="cfl"):
     return mk
 
 
-def _wave3d(real=CType.Float):
+def _wave(ndim=3, real=CType.Float):
     def mk():
         from .schemes.linear_wave import linear_wave_om
-        return linear_wave_om(3, real)
+        return linear_wave_om(ndim, real)
+    return mk
+
+
+HYDRO_FIELDS = ["density", "velocity0", "velocity1", "pressure"]
+
+
+def _hydro_adj(real):
+    def mk():
+        from .adjoint import add_adjoint
+        from .schemes.hydro import hydro_om
+        return add_adjoint(hydro_om(real, Fraction(7, 5), "fixed"), "proceed", HYDRO_FIELDS)
+    return mk
+
+
+def _wave_adj(ndim, real):
+    def mk():
+        from .adjoint import add_adjoint
+        from .schemes.linear_wave import linear_wave_om
+        return add_adjoint(linear_wave_om(ndim, real), "proceed", ["fieldF", "fieldG"])
     return mk
 
 
@@ -41,6 +60,12 @@ VARIANTS = {
     "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), False),
     "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), True),
     "hydro_f64": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    "hydro_adj_f32": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                     tile=(32, 8)), False),
+    "hydro_adj_f64": (_hydro_adj(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                      tile=(32, 8)), True),
+    "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
+    "wave3d_f64": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP, boundary=CYC3), True),
 }
 
 

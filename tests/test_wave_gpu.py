@@ -1,0 +1,72 @@
+"""Linear wave (examples-old/LinearWave/Convergence.hs restated in 3-D) on the GPU:
+the oracle, the exact discrete plane wave of the leapfrog scheme, and the
+adjoint's dot-product identity through the checkpoint driver."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle_machine
+from paraiso_amd import variants
+from paraiso_amd.adjoint import add_adjoint
+from paraiso_amd.autograd import rollout
+from paraiso_amd.om.dyn import CType
+from paraiso_amd.schemes.linear_wave import linear_wave_om
+
+pytestmark = pytest.mark.gpu
+
+
+def params(m, c=1.0, dx=1.0, dt=0.5):
+    m.set("c", c)
+    m.set("dx", dx)
+    m.set("dt", dt)
+
+
+def test_wave3d_f64_bit_exact_vs_oracle(require_gpu):
+    n = (16, 12, 10)
+    rng = np.random.default_rng(3)
+    f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
+    gpu = variants.machine("wave3d_f64", n)
+    ora = oracle_machine(lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
+                                             ["fieldF", "fieldG"]), n)
+    for m in (gpu, ora):
+        params(m)
+        m.set("fieldF", f0)
+        m.set("fieldG", g0)
+        m.run("proceed", 9)
+    assert np.array_equal(gpu.get("fieldF"), ora.get("fieldF"))
+    assert np.array_equal(gpu.get("fieldG"), ora.get("fieldG"))
+    assert abs(gpu.get("energy") - ora.get("energy")) <= 1e-12 * abs(ora.get("energy"))
+
+
+def test_wave3d_exact_plane_wave(require_gpu):
+    n = (32, 16, 8)
+    c, dx, dt, steps = 1.0, 1.0, 0.5, 40
+    k = np.array([2 * np.pi * 3 / n[0], 2 * np.pi * 1 / n[1], 2 * np.pi * 2 / n[2]])
+    s = (c * dt / dx) * np.sqrt(np.sum(np.sin(k * dx / 2) ** 2))
+    w = 2 * np.arcsin(s) / dt
+    z, y, x = np.meshgrid(*[np.arange(m, dtype=np.float64) for m in n[::-1]], indexing="ij")
+    ph = k[0] * x + k[1] * y + k[2] * z
+    f0 = np.cos(ph)
+    g0 = (np.cos(ph - w * dt) - np.cos(ph)) / dt
+    m = variants.machine("wave3d_f64", n)
+    params(m, c, dx, dt)
+    m.set("fieldF", f0)
+    m.set("fieldG", g0)
+    m.run("proceed", steps)
+    assert np.abs(m.get("fieldF") - np.cos(ph - w * dt * steps)).max() < 1e-11
+
+
+def test_wave3d_adjoint_dot_product(require_gpu):
+    n = (16, 12, 10)
+    nsteps, every = 7, 3
+    rng = np.random.default_rng(5)
+    x = [torch.tensor(rng.standard_normal(n[::-1]), device="cuda", requires_grad=True)
+         for _ in range(2)]
+    y = [torch.tensor(rng.standard_normal(n[::-1]), device="cuda") for _ in range(2)]
+    m = variants.machine("wave3d_f64", n)
+    params(m)
+    out = rollout(m, ["fieldF", "fieldG"], x, nsteps, every)
+    lhs = sum((o * yy).sum() for o, yy in zip(out, y))
+    lhs.backward()
+    rhs = sum((xx.detach() * xx.grad).sum() for xx in x)        # <x, A^T y> with A linear
+    assert abs(float(lhs) - float(rhs)) <= 1e-12 * abs(float(lhs))
