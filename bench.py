@@ -82,6 +82,111 @@ def hbm_copy_gbps():
     return 2 * n * 4 * 10 / (s.elapsed_time(e) * 1e-3) / 1e9
 
 
+def bench_batched(n_grids=256, steps=200, seed=2024):
+    """256 Hydro grids, sides log-uniform on [64, 1024], FP32, one HIP graph of `steps` steps."""
+    import numpy as np
+    import torch
+    from paraiso_amd import inputs, variants
+    sizes = [tuple(int(x) for x in s) for s in inputs.batched_sizes(n_grids, seed)]
+    b = variants.batch("hydro_f32", sizes)
+    for g, (nx, ny) in enumerate(sizes):
+        setup_hydro(b.grids[g], nx, ny, seed + g, np.float32)
+    b.run("proceed", 2, graph=False)
+    b.run("proceed", steps, graph=True)          # builds and warms the graph
+    b.sync()
+    for g, (nx, ny) in enumerate(sizes):
+        setup_hydro(b.grids[g], nx, ny, seed + g, np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run("proceed", steps, graph=True)
+    b.sync()
+    t1 = time.perf_counter()
+    cells = b.cells()
+    ms = b.run_timed("proceed", 10)
+    return {"workload": "hydro2d-kh-batched-%d-grids-fp32" % n_grids, "grids": n_grids,
+            "cells_per_step": cells, "steps": steps,
+            "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
+            "ms_per_step": (t1 - t0) / steps * 1e3, "launch_ms_per_step": [x / 10 for x in ms],
+            "sizes_seed": seed}
+
+
+def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
+    """Fixed-dt Hydro 512^2 FP32: 200 forward steps vs forward + checkpointed adjoint."""
+    import numpy as np
+    import torch
+    from paraiso_amd import inputs, variants
+    m = variants.machine("hydro_adj_f32", (n, n))
+    setup_hydro(m, n, n, 7, np.float32)
+    m.set("dt", dt)
+    names = [s[0] for s in m.statics]
+    snap = [x for x in names if not x.startswith("adj_") and not x.startswith("__")]
+    S = (steps + every - 1) // every
+    m.snapshot_alloc(S + every, snap)
+    m.run("proceed", 5)
+    m.sync()
+    setup_hydro(m, n, n, 7, np.float32)
+    m.set("dt", dt)
+    t0 = time.perf_counter()
+    m.run("proceed", steps)
+    m.sync()
+    t_fwd = time.perf_counter() - t0
+    setup_hydro(m, n, n, 7, np.float32)
+    m.set("dt", dt)
+    t0 = time.perf_counter()
+    m.adjoint_forward("proceed", steps, every)
+    m.sync()
+    t_af = time.perf_counter() - t0
+    # seed: dJ/du at the final state for J = sum over [0.25,0.75]^2 of rho |v|^2 / 2
+    rho, v0, v1 = (torch.tensor(m.get(f), device="cuda", dtype=torch.float64)
+                   for f in ("density", "velocity0", "velocity1"))
+    c = (torch.arange(n, device="cuda", dtype=torch.float64) + 0.5) / n
+    inside = ((c >= 0.25) & (c <= 0.75)).to(torch.float64)
+    mask = inside[:, None] * inside[None, :]
+    seeds = {"adj_density": 0.5 * (v0 * v0 + v1 * v1) * mask, "adj_velocity0": rho * v0 * mask,
+             "adj_velocity1": rho * v1 * mask, "adj_pressure": torch.zeros_like(rho)}
+    for k, v in seeds.items():
+        m.set(k, v.float().cpu().numpy())
+    t0 = time.perf_counter()
+    m.adjoint_backward("proceed", "proceed_adjoint", steps, every)
+    m.sync()
+    t_ab = time.perf_counter() - t0
+    g = m.get("adj_density")
+    return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "steps": steps,
+            "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
+            "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
+            "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
+            "grad_density_l1": float(np.abs(g).sum())}
+
+
+def bench_wave(n=512, steps=300):
+    """LinearWave 3-D, 512^3 FP32, c = 1, dt = 0.5 dx, Gaussian pulse (8 cells)."""
+    import numpy as np
+    import torch
+    from paraiso_amd import inputs, variants
+    m = variants.machine("wave3d_f32", (n, n, n))
+    m.set("c", 1.0)
+    m.set("dx", 1.0)
+    m.set("dt", 0.5)
+    m.set("fieldF", inputs.gaussian_pulse((n, n, n), 3, dtype=np.float32))
+    m.set("fieldG", 0.0)
+    m.run("proceed", 3)
+    m.sync()
+    e0 = m.get("energy")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = m.run_timed("proceed", steps)
+    m.sync()
+    t1 = time.perf_counter()
+    cells = n ** 3
+    dom = max(range(len(ms)), key=lambda l: ms[l])
+    achieved = cells * 16 / (ms[dom] / steps * 1e-3) / 1e9
+    return {"workload": "wave3d-%d^3-fp32" % n, "steps": steps,
+            "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
+            "ms_per_step": (t1 - t0) / steps * 1e3, "achieved_GBps": achieved,
+            "frac": achieved / HBM_PEAK_GBPS, "algorithmic_bytes_per_cell": 16,
+            "energy_drift": abs(m.get("energy") - e0) / abs(e0)}
+
+
 def cpu_baseline(nx, ny, budget_s=12.0, max_steps=60):
     """Oracle CPU machine (PlanTrans restatement, OpenMP), bounded sample."""
     import numpy as np
@@ -114,6 +219,8 @@ def main():
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, adjoint and 3-D wave measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -161,7 +268,13 @@ def main():
         out["fp32_vs_fp64_rel_l1_conserved"] = err
         out["fp32_over_fp64_speedup"] = r32["value"] / r64["value"]
     out["hbm_copy_GBps"] = hbm_copy_gbps()
-    out["fwd_adj_over_fwd"] = None
+    if not args.no_extras:
+        out["batched"] = bench_batched()
+        out["adjoint"] = bench_adjoint()
+        out["fwd_adj_over_fwd"] = out["adjoint"]["fwd_adj_over_fwd"]
+        out["wave3d"] = bench_wave()
+    else:
+        out["fwd_adj_over_fwd"] = None
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(nx, ny)
     print(json.dumps(out))
 

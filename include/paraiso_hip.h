@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PR_ABI_VERSION 1
+#define PR_ABI_VERSION 2
 
 enum pr_status {
   PR_OK = 0,
@@ -59,6 +59,20 @@ typedef struct pr_launch_ctx {
 
 typedef int (*pr_launch_fn)(const pr_launch_ctx* ctx);
 
+/* What a generated batched launcher receives: one launch over the tiles of
+ * many grids of the same machine. */
+typedef struct pr_batch_ctx {
+  const void* args;        /* device array: one packed argument block per grid */
+  const int* tile_grid;    /* device: grid of each workgroup */
+  const unsigned* tile_base; /* device: first workgroup of each grid */
+  unsigned total_tiles;
+  int n_grids;
+  void* stream;
+} pr_batch_ctx;
+
+typedef void (*pr_pack_fn)(const pr_launch_ctx* ctx, void* out);
+typedef int (*pr_batch_launch_fn)(const pr_batch_ctx* ctx);
+
 enum pr_static_flags { PR_STATIC_SCRATCH = 1 };
 
 typedef struct pr_static_desc {
@@ -71,7 +85,10 @@ typedef struct pr_static_desc {
 typedef struct pr_launch_desc {
   pr_launch_fn fn;
   int tile[3];             /* workgroup tile; 0 tiles = one workgroup (scalar-only launch) */
-  int n_reduce;            /* reductions finished by this launch's last workgroup */
+  int n_reduce;            /* reductions finished by this launch's tail kernel */
+  pr_pack_fn pack;         /* packs one grid's argument block */
+  size_t args_size;
+  pr_batch_launch_fn batched;
 } pr_launch_desc;
 
 typedef struct pr_kernel_desc {
@@ -123,6 +140,21 @@ size_t pr_static_bytes(const pr_machine* m, int s);
 /* ---- sizes (replace om_size, om_memory_size and friends, PlanTrans.hs:167-215) */
 int pr_size(const pr_machine* m, int axis);
 long long pr_cells(const pr_machine* m);
+
+/* ---- batches: many independent grids of one machine, of uneven sizes,
+ * advanced by the same launches (no counterpart in the reference).  Grid g
+ * is a full machine (pr_batch_grid) for static access; all grids share the
+ * batch's stream.  pr_batch_run_graph captures `count` calls in one HIP graph. */
+typedef struct pr_batch pr_batch;
+int pr_batch_create(const pr_machine_desc* desc, int n_grids, const int* sizes, pr_batch** out);
+int pr_batch_destroy(pr_batch* b);
+pr_machine* pr_batch_grid(pr_batch* b, int g);
+int pr_batch_run(pr_batch* b, int k, int count);
+int pr_batch_run_graph(pr_batch* b, int k, int count);
+int pr_batch_run_timed(pr_batch* b, int k, int count, double* launch_ms);
+int pr_batch_sync(pr_batch* b);
+const char* pr_batch_last_error(const pr_batch* b);
+long long pr_batch_cells(const pr_batch* b);
 
 /* ---- checkpointed adjoints (no counterpart in the reference: the reference
  * has no adjoint).  A snapshot holds copies of the listed statics.

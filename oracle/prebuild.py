@@ -26,6 +26,11 @@ def _cases():
     for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96)]:
         out.append((life_om, n, cyc2))
     out.append((lambda: hydro_om(CType.Float, G), (2048, 2048), cyc2))   # bench CPU baseline
+    from paraiso_amd.adjoint import add_adjoint
+    from paraiso_amd.schemes.linear_wave import linear_wave_om
+    cyc3 = (A.Condition.Cyclic,) * 3
+    out.append((lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
+                                    ["fieldF", "fieldG"]), (16, 12, 10), cyc3))
     for n in [(64, 48), (96, 64), (48, 32)]:
         out.append((lambda: hydro_om(CType.Double, G), n, cyc2))
         out.append((lambda: hydro_om(CType.Float, G), n, cyc2))

@@ -1,0 +1,163 @@
+"""Tile-shape autotuner for generated machines.
+
+Replaces the reference's genetic tuner (``Language/Paraiso/Tuning/Genetic.hs``,
+``examples-old/GA``), whose genome was the CUDA grid size plus Manifest and
+``__syncthreads`` bits and whose fitness was cells*steps/second.  Here the
+search space is the workgroup tile of the fused launches (which fixes halo
+overhead, LDS footprint and hence occupancy), the fitness is measured device
+time per step (HIP events on the machine's stream), and the search is
+exhaustive over a small candidate set -- each candidate is a separate
+compilation, so candidates are compiled ahead of time on the host and only
+timed on the GPU.
+
+    python -m paraiso_amd.autotune prepare --variant hydro_f32
+    python -m paraiso_amd.autotune measure --variant hydro_f32 --size 2048 2048 --out tune.json
+    python -m paraiso_amd.autotune select tune.json          # records paraiso_amd/tuned.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from typing import Dict, List, Tuple
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNED = os.path.join(HERE, "tuned.json")
+
+# (tile, work-items per workgroup).  More work-items per workgroup raise the LDS
+# budget, so larger tiles keep their intermediates in LDS and recompute less halo.
+CANDIDATES = {
+    2: [((64, 8), 256), ((32, 8), 256), ((32, 16), 256), ((64, 4), 256), ((128, 4), 256),
+        ((64, 16), 256), ((32, 32), 256), ((32, 16), 512), ((64, 16), 512), ((32, 32), 512),
+        ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024)],
+    3: [((32, 4, 4), 256), ((64, 2, 2), 256), ((64, 4, 2), 256), ((64, 4, 4), 256),
+        ((32, 8, 4), 256), ((64, 8, 2), 256), ((32, 8, 8), 512), ((64, 4, 4), 512),
+        ((64, 8, 4), 512)],
+}
+
+
+def tuned() -> Dict:
+    if os.path.exists(TUNED):
+        with open(TUNED) as f:
+            return json.load(f)
+    return {}
+
+
+def candidate_name(variant: str, tile, threads: int = 256) -> str:
+    return "%s__t%s_n%d" % (variant, "x".join(str(t) for t in tile), threads)
+
+
+def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[int, ...], int, str]]:
+    from concurrent.futures import ThreadPoolExecutor
+    from . import variants
+    from .generator import build_machine
+    mk, setup, precise = variants.VARIANTS[variant]
+    cands = candidates or CANDIDATES[setup.ndim]
+
+    def one(c):
+        tile, nt = tuple(c[0]), int(c[1])
+        name = candidate_name(variant, tile, nt)
+        return tile, nt, build_machine(setup.with_(tile=tile, threads=nt), mk(), name,
+                                       precise=precise)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        return list(ex.map(one, cands))
+
+
+def _setup_inputs(variant: str, m, size):
+    import numpy as np
+    from . import inputs
+    if variant.startswith("hydro"):
+        nx, ny = size
+        dt = np.float32 if "f32" in variant else np.float64
+        for k, v in inputs.hydro_scalars(nx, ny).items():
+            m.set(k, v)
+        if "adj" in variant:
+            m.set("dt", 1e-4)
+        for name, arr in zip(("density", "velocity0", "velocity1", "pressure"),
+                             inputs.kh_init(nx, ny, 1, dtype=dt)):
+            m.set(name, arr)
+    elif variant.startswith("wave"):
+        dt = np.float32 if "f32" in variant else np.float64
+        m.set("c", 1.0)
+        m.set("dx", 1.0)
+        m.set("dt", 0.5)
+        m.set("fieldF", inputs.gaussian_pulse(tuple(reversed(size)), 1, dtype=dt))
+        m.set("fieldG", 0.0)
+    elif variant.startswith("life"):
+        m.set("cell", inputs.life_init(tuple(reversed(size)), 1))
+    else:
+        raise KeyError(variant)
+
+
+def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=None,
+            kernel: str = "proceed") -> List[Dict]:
+    from . import variants
+    from .generator import GEN_DIR
+    from .machine import Machine
+    mk, setup, _ = variants.VARIANTS[variant]
+    cands = candidates or CANDIDATES[setup.ndim]
+    results = []
+    for tile, nt in cands:
+        name = candidate_name(variant, tile, nt)
+        so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
+        if not os.path.exists(so):
+            continue
+        print("tune %s tile=%s threads=%d" % (variant, tile, nt), flush=True)
+        m = Machine(so, size)
+        _setup_inputs(variant, m, size)
+        m.run(kernel, warmup)
+        m.sync()
+        ms = m.run_timed(kernel, steps)
+        results.append({"tile": list(tile), "threads": nt, "ms_per_step": sum(ms) / steps,
+                        "launch_ms": [x / steps for x in ms]})
+        m.close()
+    return results
+
+
+def select(path: str) -> Dict:
+    with open(path) as f:
+        data = json.load(f)
+    t = tuned()
+    for variant, rec in data.items():
+        best = min(rec["results"], key=lambda r: r["ms_per_step"])
+        t[variant] = {"tile": best["tile"], "threads": best["threads"],
+                      "ms_per_step": best["ms_per_step"],
+                      "size": rec["size"], "steps": rec["steps"],
+                      "candidates": {"%s_n%d" % ("x".join(map(str, r["tile"])), r["threads"]):
+                                     round(r["ms_per_step"], 5) for r in rec["results"]}}
+    with open(TUNED, "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+    return t
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("prepare")
+    p.add_argument("--variant", required=True)
+    q = sub.add_parser("measure")
+    q.add_argument("--variant", action="append", required=True)
+    q.add_argument("--size", type=int, nargs="+", action="append", required=True)
+    q.add_argument("--steps", type=int, default=50)
+    q.add_argument("--out", required=True)
+    s = sub.add_parser("select")
+    s.add_argument("path")
+    a = ap.parse_args(argv)
+    if a.cmd == "prepare":
+        for tile, nt, so in prepare(a.variant):
+            print(tile, nt, so)
+    elif a.cmd == "measure":
+        out = {}
+        for variant, size in zip(a.variant, a.size):
+            out[variant] = {"size": size, "steps": a.steps,
+                            "results": measure(variant, size, a.steps)}
+            with open(a.out, "w") as f:
+                json.dump(out, f, indent=1)
+    else:
+        print(json.dumps(select(a.path), indent=1))
+
+
+if __name__ == "__main__":
+    main()

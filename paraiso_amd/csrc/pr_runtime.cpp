@@ -18,6 +18,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -466,5 +467,270 @@ int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every) 
   }
   return PR_OK;
 }
+
+
+// ---------------------------------------------------------------- batches
+}  // extern "C"
+
+struct pr_batch {
+  const pr_machine_desc* d = nullptr;
+  std::vector<pr_machine*> grids;
+  hipStream_t stream = nullptr;
+  long long cells = 0;
+  // per (kernel, launch): grid table
+  std::vector<std::vector<int*>> tile_grid;
+  std::vector<std::vector<unsigned*>> tile_base;
+  std::vector<std::vector<unsigned>> total;
+  // per (kernel, launch, parity mask): packed argument blocks on the device
+  std::map<std::tuple<int, int, unsigned long long>, char*> args;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::string err;
+};
+
+namespace {
+
+int bfail(pr_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+// Build the launch context of grid machine m for launch (k, l), as launch_kernel_once does.
+void grid_ctx(pr_machine* m, int k, int l, std::vector<const void*>& cur, std::vector<void*>& next,
+              pr_launch_ctx& ctx) {
+  const int ns = m->d->n_static;
+  cur.assign(ns, nullptr);
+  next.assign(ns, nullptr);
+  for (int s = 0; s < ns; ++s) {
+    if (m->d->statics[s].realm != PR_ARRAY) continue;
+    cur[s] = m->buf[m->cur[s]][s];
+    if (m->stored_any[s]) next[s] = m->buf[1 - m->cur[s]][s];
+  }
+  ctx.cur = cur.data();
+  ctx.next = next.data();
+  ctx.scal = m->scal;
+  ctx.part = m->part + m->part_off[k][l];
+  ctx.counter = reinterpret_cast<unsigned*>(m->scal + 8 * m->counter_idx[k][l]);
+  for (int a = 0; a < 3; ++a) {
+    ctx.n[a] = m->n[a];
+    ctx.tiles[a] = m->tiles[k][l][a];
+  }
+  ctx.ntiles = m->ntiles[k][l];
+  ctx.stream = m->stream;
+}
+
+// Device block of packed arguments for (k, l) at the grids' current parity; built on first use
+// with synchronous copies, so a run that is captured must have built it beforehand.
+int batch_args(pr_batch* b, int k, int l, char** out) {
+  const auto key = std::make_tuple(k, l, parity_mask(b->grids[0]));
+  auto it = b->args.find(key);
+  if (it != b->args.end()) {
+    *out = it->second;
+    return PR_OK;
+  }
+  const pr_launch_desc& ld = b->d->kernels[k].launches[l];
+  const size_t sz = ld.args_size;
+  std::vector<char> host(sz * b->grids.size());
+  std::vector<const void*> cur;
+  std::vector<void*> next;
+  for (size_t g = 0; g < b->grids.size(); ++g) {
+    pr_launch_ctx ctx;
+    grid_ctx(b->grids[g], k, l, cur, next, ctx);
+    ld.pack(&ctx, host.data() + g * sz);
+  }
+  char* dev = nullptr;
+  if (hipMalloc(&dev, host.size()) != hipSuccess) return bfail(b, PR_E_NOMEM, "batch args");
+  if (hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return bfail(b, PR_E_HIP, "batch args copy");
+  b->args[key] = dev;
+  *out = dev;
+  return PR_OK;
+}
+
+int batch_once(pr_batch* b, int k, double* launch_ms, std::vector<hipEvent_t>* ev) {
+  const pr_kernel_desc& kd = b->d->kernels[k];
+  for (int l = 0; l < kd.n_launch; ++l) {
+    char* args = nullptr;
+    int rc = batch_args(b, k, l, &args);
+    if (rc) return rc;
+    pr_batch_ctx bc;
+    bc.args = args;
+    bc.tile_grid = b->tile_grid[k][l];
+    bc.tile_base = b->tile_base[k][l];
+    bc.total_tiles = b->total[k][l];
+    bc.n_grids = (int)b->grids.size();
+    bc.stream = b->stream;
+    if (ev) (void)hipEventRecord((*ev)[2 * l], b->stream);
+    if (kd.launches[l].batched(&bc) != 0)
+      return bfail(b, PR_E_HIP, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
+    if (ev) (void)hipEventRecord((*ev)[2 * l + 1], b->stream);
+  }
+  for (pr_machine* m : b->grids)
+    for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+  return PR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pr_batch_create(const pr_machine_desc* d, int n_grids, const int* sizes, pr_batch** out) {
+  if (!d || n_grids < 1 || !sizes || !out) return PR_E_INVALID;
+  if (d->abi != PR_ABI_VERSION) return PR_E_ABI;
+  *out = nullptr;
+  for (int k = 0; k < d->n_kernel; ++k)
+    for (int l = 0; l < d->kernels[k].n_launch; ++l)
+      if (!d->kernels[k].launches[l].batched || !d->kernels[k].launches[l].pack)
+        return PR_E_UNSUPPORTED;   // generated without Setup.batched
+  pr_batch* b = new pr_batch();
+  b->d = d;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return PR_E_HIP;
+  }
+  for (int g = 0; g < n_grids; ++g) {
+    pr_machine* m = nullptr;
+    int rc = pr_create(d, sizes + 3 * g, &m);
+    if (rc == PR_OK) rc = pr_set_stream(m, b->stream);
+    if (rc) {
+      b->err = "grid " + std::to_string(g) + ": " + (m ? m->err : std::string("pr_create"));
+      pr_batch_destroy(b);
+      return rc;
+    }
+    b->grids.push_back(m);
+    b->cells += m->cells;
+  }
+  b->tile_grid.resize(d->n_kernel);
+  b->tile_base.resize(d->n_kernel);
+  b->total.resize(d->n_kernel);
+  for (int k = 0; k < d->n_kernel; ++k) {
+    for (int l = 0; l < d->kernels[k].n_launch; ++l) {
+      std::vector<int> tg;
+      std::vector<unsigned> tb;
+      for (int g = 0; g < n_grids; ++g) {
+        tb.push_back((unsigned)tg.size());
+        for (unsigned t = 0; t < b->grids[g]->ntiles[k][l]; ++t) tg.push_back(g);
+      }
+      int* dtg = nullptr;
+      unsigned* dtb = nullptr;
+      if (hipMalloc(&dtg, tg.size() * sizeof(int)) != hipSuccess ||
+          hipMalloc(&dtb, tb.size() * sizeof(unsigned)) != hipSuccess ||
+          hipMemcpy(dtg, tg.data(), tg.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(dtb, tb.data(), tb.size() * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+        pr_batch_destroy(b);
+        return PR_E_NOMEM;
+      }
+      b->tile_grid[k].push_back(dtg);
+      b->tile_base[k].push_back(dtb);
+      b->total[k].push_back((unsigned)tg.size());
+    }
+  }
+  *out = b;
+  return PR_OK;
+}
+
+int pr_batch_destroy(pr_batch* b) {
+  if (!b) return PR_E_INVALID;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : b->args) (void)hipFree(kv.second);
+  for (auto& v : b->tile_grid)
+    for (int* p : v) (void)hipFree(p);
+  for (auto& v : b->tile_base)
+    for (unsigned* p : v) (void)hipFree(p);
+  for (pr_machine* m : b->grids) pr_destroy(m);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PR_OK;
+}
+
+pr_machine* pr_batch_grid(pr_batch* b, int g) {
+  if (!b || g < 0 || g >= (int)b->grids.size()) return nullptr;
+  return b->grids[g];
+}
+
+int pr_batch_run(pr_batch* b, int k, int count) {
+  if (!b || k < 0 || k >= b->d->n_kernel || count < 0) return PR_E_INVALID;
+  for (int c = 0; c < count; ++c) {
+    int rc = batch_once(b, k, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return PR_OK;
+}
+
+int pr_batch_run_timed(pr_batch* b, int k, int count, double* launch_ms) {
+  if (!b || k < 0 || k >= b->d->n_kernel || count < 0 || !launch_ms) return PR_E_INVALID;
+  const int nl = b->d->kernels[k].n_launch;
+  std::vector<std::vector<hipEvent_t>> evs(count, std::vector<hipEvent_t>(2 * nl));
+  for (auto& v : evs)
+    for (auto& e : v)
+      if (hipEventCreate(&e) != hipSuccess) return bfail(b, PR_E_HIP, "hipEventCreate");
+  int rc = PR_OK;
+  for (int c = 0; c < count && rc == PR_OK; ++c) rc = batch_once(b, k, nullptr, &evs[c]);
+  if (rc == PR_OK) {
+    (void)hipStreamSynchronize(b->stream);
+    for (int l = 0; l < nl; ++l) launch_ms[l] = 0;
+    for (int c = 0; c < count; ++c)
+      for (int l = 0; l < nl; ++l) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, evs[c][2 * l], evs[c][2 * l + 1]);
+        launch_ms[l] += ms;
+      }
+  }
+  for (auto& v : evs)
+    for (auto& e : v) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int pr_batch_run_graph(pr_batch* b, int k, int count) {
+  if (!b || k < 0 || k >= b->d->n_kernel || count < 1) return PR_E_INVALID;
+  pr_machine* m0 = b->grids[0];
+  GraphKey key{k, count, parity_mask(m0)};
+  auto it = b->graphs.find(key);
+  if (it == b->graphs.end()) {
+    // build every argument block the captured sequence needs before capturing
+    std::vector<std::vector<int>> saved;
+    for (pr_machine* m : b->grids) saved.push_back(m->cur);
+    const pr_kernel_desc& kd = b->d->kernels[k];
+    for (int c = 0; c < count && c < 2; ++c) {
+      for (int l = 0; l < kd.n_launch; ++l) {
+        char* a;
+        int rc = batch_args(b, k, l, &a);
+        if (rc) return rc;
+      }
+      for (pr_machine* m : b->grids)
+        for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+    }
+    for (size_t g = 0; g < b->grids.size(); ++g) b->grids[g]->cur = saved[g];
+    hipGraph_t gr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return bfail(b, PR_E_HIP, "begin capture");
+    int rc = PR_OK;
+    for (int c = 0; c < count && rc == PR_OK; ++c) rc = batch_once(b, k, nullptr, nullptr);
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    for (size_t g = 0; g < b->grids.size(); ++g) b->grids[g]->cur = saved[g];
+    if (rc) return rc;
+    if (e != hipSuccess) return bfail(b, PR_E_HIP, "end capture");
+    hipGraphExec_t ex;
+    e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return bfail(b, PR_E_HIP, "graph instantiate");
+    it = b->graphs.emplace(key, ex).first;
+  }
+  if (hipGraphLaunch(it->second, b->stream) != hipSuccess) return bfail(b, PR_E_HIP, "graph launch");
+  const pr_kernel_desc& kd = b->d->kernels[k];
+  if (count % 2)
+    for (pr_machine* m : b->grids)
+      for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+  return PR_OK;
+}
+
+int pr_batch_sync(pr_batch* b) {
+  if (!b) return PR_E_INVALID;
+  return hipStreamSynchronize(b->stream) == hipSuccess ? PR_OK : PR_E_HIP;
+}
+
+const char* pr_batch_last_error(const pr_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+long long pr_batch_cells(const pr_batch* b) { return b ? b->cells : 0; }
 
 }  // extern "C"

@@ -33,11 +33,11 @@ HIPCC_BASE = ["hipcc", "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std
               "-Wno-unused-variable", "-Wno-unused-result", "-Wno-unused-value"]
 
 
-def _hip_sources(setup: Setup, om: OM) -> Tuple[Plan, str, str]:
+def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str, str]:
     from .hip.emit import MachineEmitter
     from .hip.host import emit_host_class
     from .hip.ir import lower_kernel
-    from .hip.schedule import schedule_kernel
+    from .hip.schedule import lds_budget, schedule_kernel
     if any(b is not A.Condition.Cyclic for b in setup.boundary):
         raise NotImplementedError("the HIP backend lowers Cyclic boundaries; Open margins are not "
                                   "implemented for the GPU path")
@@ -45,9 +45,12 @@ def _hip_sources(setup: Setup, om: OM) -> Tuple[Plan, str, str]:
     om_opt = plan.om
     tile = tuple(setup.tile) if setup.tile else DEFAULT_TILES[om_opt.ndim]
     cyclic = tuple(b is A.Condition.Cyclic for b in setup.boundary)
-    scheds = [schedule_kernel(lower_kernel(om_opt, ki, cyclic), tile)
+    budget = lds_budget(setup.threads)
+    scheds = [schedule_kernel(lower_kernel(om_opt, ki, cyclic), tile, budget)
               for ki in range(len(om_opt.kernels))]
-    src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile).emit()
+    src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,
+                         fast_fp32=not precise, batched=setup.batched,
+                         threads=setup.threads).emit()
     hpp = emit_host_class(om_opt, setup)
     return plan, src, hpp
 
@@ -94,14 +97,16 @@ def build_machine(setup: Setup, om: OM, variant: str, precise: bool = False,
 
     ``precise`` compiles without FP contraction and with correctly rounded
     FP32 division and square root (bit-comparable with the CPU oracle);
-    the default contracts multiply-adds and uses the fast FP32 reciprocal.
+    otherwise multiply-adds are contracted and FP32 division and square root
+    use the hardware reciprocal and square root (about 1 ulp).
     """
-    _, src, hpp = _hip_sources(setup, om)
+    _, src, hpp = _hip_sources(setup, om, precise)
     flags = list(HIPCC_BASE) + ["-I" + CSRC_DIR, "-I" + INCLUDE_DIR]
     if precise:
         flags += ["-ffp-contract=off"]
     else:
-        flags += ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+        # FP32 division and square root use v_rcp_f32 / v_sqrt_f32 in the generated source
+        flags += ["-ffp-contract=fast"]
     d = os.path.join(GEN_DIR, variant)
     os.makedirs(d, exist_ok=True)
     hip_path = os.path.join(d, variant + ".hip")

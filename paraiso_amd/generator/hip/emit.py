@@ -70,6 +70,9 @@ def acc_op(op: ReduceOp, ct: CType) -> str:
     return "OmMin()" if op is ReduceOp.Min else "OmMax()"
 
 
+FAST_FP32 = {"on": False}      # set per machine by MachineEmitter
+
+
 def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType) -> str:
     infix = {Op.Add: "+", Op.Sub: "-", Op.Mul: "*", Op.Div: "/", Op.Mod: "%",
              Op.And: "&&", Op.Or: "||", Op.EQ: "==", Op.NE: "!=", Op.LT: "<",
@@ -77,11 +80,16 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
     t0 = arg_types[0] if arg_types else out
     if op is Op.Identity:
         return args[0]
+    if op is Op.Div and t0 is CType.Float and FAST_FP32["on"]:
+        # v_rcp_f32 (1 ulp) and a multiply instead of the scaled IEEE division
+        return "%s * __builtin_amdgcn_rcpf(%s)" % (_p(args[0]), args[1])
     if op in infix:
         return "%s %s %s" % (_p(args[0]), infix[op], _p(args[1]))
     if op is Op.Neg:
         return "-" + _p(args[0])
     if op is Op.Inv:
+        if t0 is CType.Float and FAST_FP32["on"]:
+            return "__builtin_amdgcn_rcpf(%s)" % args[0]
         return "1/" + _p(args[0])
     if op is Op.Not:
         return "!" + _p(args[0])
@@ -100,6 +108,8 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
     if op is Op.Signum:
         return "(%s)(((%s) > 0) - ((%s) < 0))" % (out.c_name, args[0], args[0])
     if op is Op.Sqrt:
+        if t0 is CType.Float and FAST_FP32["on"]:
+            return "__builtin_amdgcn_sqrtf(%s)" % args[0]
         return ("sqrtf(%s)" if t0 is CType.Float else "sqrt(%s)") % args[0]
     if op in (Op.Exp, Op.Log, Op.Sin, Op.Cos, Op.Tan, Op.Asin, Op.Acos, Op.Atan):
         f = op.value.lower()
@@ -118,8 +128,12 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
 
 class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
-                 tile: Tuple[int, ...]):
+                 tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
+                 threads: int = NT):
         self.name = name
+        self.nt = threads
+        self.fast_fp32 = fast_fp32
+        self.batched = batched
         self.statics = statics           # list of Named
         self.ndim = ndim
         self.schedules = schedules
@@ -190,19 +204,17 @@ class MachineEmitter:
                    "int n0, n1, n2;", "int t0, t1, t2;", "unsigned ntiles;"]
         struct = "struct Args_%s {\n  %s\n};\n" % (fn, "\n  ".join(fields))
 
-        body: List[str] = []
-        w = body.append
-        w("extern \"C\" __global__ void __launch_bounds__(%d) %s(Args_%s a) {" % (NT, fn, fn))
+        decls: List[str] = []
         if la.lds_bytes:
-            w("  __shared__ __align__(16) char smem[%d];" % la.lds_bytes)
+            decls.append("  __shared__ __align__(16) char smem[%d];" % la.lds_bytes)
         acc_types = set()
         for r in la.reduces:
             acc_types.add(acc_type(nodes[r].payload, nodes[nodes[r].args[0]].ctype))
         for at in sorted(acc_types, key=lambda c: c.value):
-            w("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
-        # tile coordinates
-        w("  unsigned tile = blockIdx.x;")
-        w("  if ((a.ntiles & 7u) == 0u) tile = (tile & 7u) * (a.ntiles >> 3) + (tile >> 3);")
+            decls.append("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
+        body: List[str] = []
+        w = body.append
+        # tile coordinates (``tile`` is set by the single-grid or the batched prologue)
         w("  const int X0 = (int)(tile %% (unsigned)a.t0) * %d;" % T[0])
         w("  const int X1 = (int)((tile / (unsigned)a.t0) %% (unsigned)a.t1) * %d;" % T[1])
         w("  const int X2 = (int)(tile / ((unsigned)a.t0 * (unsigned)a.t1)) * %d;" % T[2])
@@ -237,15 +249,11 @@ class MachineEmitter:
                 at = acc_type(nodes[r].payload, a_ct)
                 w("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
                     at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
-                w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + blockIdx.x] = br; }" % (
+                w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + tile] = br; }" % (
                     at.c_name, j))
-        w("}")
         fin = []
         if needs_tail:
             f = fin.append
-            f("extern \"C\" __global__ void __launch_bounds__(%d) %s_fin(Args_%s a) {" % (NT, fn, fn))
-            for at in sorted(acc_types, key=lambda c: c.value):
-                f("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
             for n in nodes:
                 if n.realm is not Realm.Scalar:
                     continue
@@ -272,14 +280,47 @@ class MachineEmitter:
                 for s_, v in sch.scalar_stores:
                     f("    *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s_, v))
             f("  }")
-            f("}")
-        body.extend(fin)
-        kernel_src = struct + "\n".join(body) + "\n"
+        fin_decls = ["  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name)
+                     for at in sorted(acc_types, key=lambda c: c.value)]
+        ks = [struct]
+        core = "\n".join(body)
+        ks.append("extern \"C\" __global__ void __launch_bounds__(%d) %s(Args_%s a) {" % (self.nt, fn, fn))
+        ks.extend(decls)
+        ks.append("  unsigned tile = blockIdx.x;")
+        ks.append("  if ((a.ntiles & 7u) == 0u) tile = (tile & 7u) * (a.ntiles >> 3) + (tile >> 3);")
+        ks.append(core)
+        ks.append("}")
+        # batched form: the tiles of many grids in one launch (grid table in device memory)
+        if self.batched:
+            ks.append("extern \"C\" __global__ void __launch_bounds__(%d) %s_batched(const Args_%s* __restrict__ grids, "
+                      "const int* __restrict__ tile_grid, const unsigned* __restrict__ tile_base, unsigned total) {"
+                      % (self.nt, fn, fn))
+            ks.extend(decls)
+            ks.append("  unsigned b = blockIdx.x;")
+            ks.append("  if ((total & 7u) == 0u) b = (b & 7u) * (total >> 3) + (b >> 3);")
+            ks.append("  const int g = tile_grid[b];")
+            ks.append("  const Args_%s a = grids[g];" % fn)
+            ks.append("  const unsigned tile = b - tile_base[g];")
+            ks.append(core)
+            ks.append("}")
+        if needs_tail:
+            fcore = "\n".join(fin)
+            ks.append("extern \"C\" __global__ void __launch_bounds__(%d) %s_fin(Args_%s a) {" % (self.nt, fn, fn))
+            ks.extend(fin_decls)
+            ks.append(fcore)
+            ks.append("}")
+            if self.batched:
+                ks.append("extern \"C\" __global__ void __launch_bounds__(%d) %s_fin_batched(const Args_%s* __restrict__ grids) {"
+                          % (self.nt, fn, fn))
+                ks.extend(fin_decls)
+                ks.append("  const Args_%s a = grids[blockIdx.x];" % fn)
+                ks.append(fcore)
+                ks.append("}")
+        kernel_src = "\n".join(ks) + "\n"
 
         # ---- host launcher
         h = []
-        h.append("static int launch_%s(const pr_launch_ctx* c) {" % fn)
-        h.append("  Args_%s a;" % fn)
+        h.append("static void fill_%s(const pr_launch_ctx* c, Args_%s& a) {" % (fn, fn))
         for s in loads:
             h.append("  a.ld%d = (const %s*)c->cur[%d];" % (s, self.statics[s].dyn.ctype.c_name, s))
         for s in stores:
@@ -288,12 +329,30 @@ class MachineEmitter:
         h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
         h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
         h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
+        h.append("}")
+        h.append("static void pack_%s(const pr_launch_ctx* c, void* out) {" % fn)
+        h.append("  Args_%s a; fill_%s(c, a); memcpy(out, &a, sizeof a);" % (fn, fn))
+        h.append("}")
+        h.append("static int launch_batched_%s(const pr_batch_ctx* bc) {" % fn)
+        if not self.batched:
+            h.append("  (void)bc; return -1;")
+        elif has_work:
+            h.append("  hipLaunchKernelGGL(%s_batched, dim3(bc->total_tiles), dim3(%d), 0, (hipStream_t)bc->stream, "
+                     "(const Args_%s*)bc->args, bc->tile_grid, bc->tile_base, bc->total_tiles);" % (fn, self.nt, fn))
+        if needs_tail and self.batched:
+            h.append("  hipLaunchKernelGGL(%s_fin_batched, dim3(bc->n_grids), dim3(%d), 0, (hipStream_t)bc->stream, "
+                     "(const Args_%s*)bc->args);" % (fn, self.nt, fn))
+        if self.batched:
+            h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
+        h.append("}")
+        h.append("static int launch_%s(const pr_launch_ctx* c) {" % fn)
+        h.append("  Args_%s a; fill_%s(c, a);" % (fn, fn))
         if has_work:
             h.append("  hipLaunchKernelGGL(%s, dim3(c->ntiles), dim3(%d), 0, (hipStream_t)c->stream, a);"
-                     % (fn, NT))
+                     % (fn, self.nt))
         if needs_tail:
             h.append("  hipLaunchKernelGGL(%s_fin, dim3(1), dim3(%d), 0, (hipStream_t)c->stream, a);"
-                     % (fn, NT))
+                     % (fn, self.nt))
         h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"
@@ -309,7 +368,7 @@ class MachineEmitter:
         w("  // phase %d: %s over box %s" % (ph.index, "store" if is_out else
                                              "values " + ",".join("n%d" % v for v in ph.nodes),
                                              list(box[:nd])))
-        w("  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, NT))
+        w("  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, self.nt))
         w("    const int l0 = p %% %d;" % W[0])
         w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
         w("    const int l2 = p / %d;" % (W[0] * W[1]))
@@ -429,8 +488,17 @@ class MachineEmitter:
 
     # ------------------------------------------------------------ machine
     def emit(self) -> str:
+        FAST_FP32["on"] = self.fast_fp32
+        try:
+            return self._emit()
+        finally:
+            FAST_FP32["on"] = False
+
+    def _emit(self) -> str:
         out = ["// Generated by paraiso_amd for Orthotope Machine '%s' (gfx950)." % self.name,
-               "#include <math.h>", "#include \"om_device.h\"", "#include \"paraiso_hip.h\"", ""]
+               "#include <math.h>", "#include <string.h>", "#define OM_NT %d" % self.nt,
+               "#include \"om_device.h\"",
+               "#include \"paraiso_hip.h\"", ""]
         launchers = []
         kdescs = []
         for ki, sch in enumerate(self.schedules):
@@ -440,8 +508,14 @@ class MachineEmitter:
                 out.append(ksrc)
                 launchers.append(hsrc)
                 tile = list(self.tile) if la.targets else [0, 0, 0]
-                ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d}" % (
-                    ki, la.index, tile[0], tile[1], tile[2], len(la.reduces)))
+                if self.batched:
+                    ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d, pack_om_k%d_l%d, sizeof(Args_om_k%d_l%d), "
+                                 "launch_batched_om_k%d_l%d}" % (
+                                     ki, la.index, tile[0], tile[1], tile[2], len(la.reduces),
+                                     ki, la.index, ki, la.index, ki, la.index))
+                else:
+                    ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d, nullptr, 0, nullptr}" % (
+                        ki, la.index, tile[0], tile[1], tile[2], len(la.reduces)))
             stored = sorted(set(s for s, _ in sch.array_stores))
             out_l = "static const pr_launch_desc k_launch_%d[] = {%s};" % (ki, ", ".join(ldesc))
             launchers.append(out_l)

@@ -32,7 +32,16 @@ from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
-LDS_BUDGET = 64 * 1024        # bytes per workgroup; larger launches are cut (two workgroups per CU)
+LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches are cut
+LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
+
+
+def lds_budget(threads: int) -> int:
+    """LDS bytes a workgroup of ``threads`` may use: two resident workgroups per CU up to
+    512 work-items (one can compute while the other waits at a barrier), one above."""
+    if threads <= 256:
+        return LDS_BUDGET
+    return LDS_MAX // 2 if threads <= 512 else LDS_MAX
 
 
 @dataclass
@@ -104,7 +113,7 @@ def _neg(v):
     return tuple(-x for x in v)
 
 
-def schedule_kernel(k: LKernel, tile: Tuple[int, ...]) -> KernelSchedule:
+def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET) -> KernelSchedule:
     nd = k.ndim
     zero = tuple(0 for _ in range(nd))
     nodes = k.nodes
@@ -140,7 +149,7 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...]) -> KernelSchedule:
         red_L = [r for r in reduces if level[nodes[r].args[0]] == L]
         tail = [s for s in slot_scalars if level[s] == L + 1]
         launches.extend(_schedule_level(k, L, targets, stores_L, red_L, tail,
-                                        L == n_launch - 1, costs, tile, zero, scratch))
+                                        L == n_launch - 1, costs, tile, zero, scratch, budget))
     for i, la in enumerate(launches):
         la.index = i
     halo = 0
@@ -161,15 +170,16 @@ SCRATCH_BASE = 1 << 20        # scratch arrays are numbered from here until the 
 
 
 def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, scratch,
-                    depth=0) -> List[ArrayLaunch]:
+                    budget, depth=0) -> List[ArrayLaunch]:
     """Schedule one level; cut it into consecutive launches while LDS is over budget.
 
     A cut after phase q stores every materialised value computed at or before
     q and read after it into a scratch array (its exact bits); the second
     launch reads those arrays as halo tiles, like loads of statics."""
-    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero)
+    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, budget)
+    la.kernel_override = k
     inner = [ph for ph in la.phases[:-1] if ph.index > 0]
-    if la.lds_bytes <= LDS_BUDGET or len(inner) < 2 or depth > 6:
+    if la.lds_bytes <= budget or len(inner) < 2 or depth > 6:
         return [la]
     nodes = k.nodes
     # cut where half of the computed materialised values are behind us
@@ -199,21 +209,19 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, z
         scratch.append((v, nodes[v].ctype))
         first_stores.append((sidx, v))
     first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, costs, tile, zero,
-                            scratch, depth + 1)
+                            scratch, budget, depth + 1)
     # second launch: the same targets, with the cut values read back as loads
     sidx_of = {v: s for s, v in first_stores}
     nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of else n
               for n in nodes]
     k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
     second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, costs, tile, zero,
-                             scratch, depth + 1)
-    for la2 in second:
-        la2.kernel_override = k2
+                             scratch, budget, depth + 1)
     return first + second
 
 
 def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, costs,
-                     tile, zero) -> ArrayLaunch:
+                     tile, zero, budget=LDS_BUDGET) -> ArrayLaunch:
     nodes = k.nodes
     nd = k.ndim
     # cone of array values needed by this launch
@@ -254,6 +262,25 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                 if n.op in ("arith", "shift") and n.realm is Realm.Array:
                     st.extend(n.args)
             return out
+        def reads_with_cursors(root):
+            out: Dict[int, Set] = {}
+            seen = set()
+            st = [(root, zero)]
+            while st:
+                x, c = st.pop()
+                if (x, c) in seen:
+                    continue
+                seen.add((x, c))
+                if x in M and x != root:
+                    out.setdefault(x, set()).add(c)
+                    continue
+                n = nodes[x]
+                if n.op == "shift":
+                    st.append((n.args[0], _add(c, _neg(n.payload))))
+                elif n.op == "arith" and n.realm is Realm.Array:
+                    st.extend((a, c) for a in n.args if nodes[a].realm is Realm.Array)
+            return out
+
         phase: Dict[int, int] = {}
         for v in sorted(M):
             if v in loads:
@@ -261,8 +288,15 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         for v in sorted(M):
             if v in loads:
                 continue
-            r = reads_of([v])
-            phase[v] = 1 + max([phase[u] for u in r] + [0])
+            # a value read only at offset 0 is available in registers at the same
+            # point of the same loop, so it may share the reader's phase
+            p = 1
+            for u, curs in reads_with_cursors(v).items():
+                if u in loads or curs != {zero}:
+                    p = max(p, phase[u] + 1)
+                else:
+                    p = max(p, phase[u])
+            phase[v] = p
         out_roots = [t for t in targets if t not in M]
         r_out = reads_of(out_roots) | set(t for t in targets if t in M)
         out_phase = 1 + max([phase[u] for u in r_out] + [0])
@@ -377,8 +411,8 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         added = False
         for gain, v in ranked[:4]:
             trial = set(M) | {v}
-            if layout(trial)[6] <= LDS_BUDGET or layout(M)[6] > LDS_BUDGET:
-                if layout(trial)[6] <= max(LDS_BUDGET, layout(M)[6]):
+            if layout(trial)[6] <= budget or layout(M)[6] > budget:
+                if layout(trial)[6] <= max(budget, layout(M)[6]):
                     M = trial
                     added = True
                     break

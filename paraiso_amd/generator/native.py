@@ -30,6 +30,8 @@ class Setup:
     # HIP-only knobs (tile shape, temporal unroll); None lets the generator choose
     tile: Optional[Tuple[int, ...]] = None
     steps_per_launch: int = 1
+    batched: bool = False        # also emit the many-grids form of every launch
+    threads: int = 256           # work-items per workgroup (a multiple of 64, at most 1024)
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)
@@ -38,6 +40,8 @@ class Setup:
         self.boundary = tuple(self.boundary)
         if len(self.boundary) != len(self.local_size):
             raise ValueError("boundary and local_size differ in dimension")
+        if self.threads % 64 or not 64 <= self.threads <= 1024:
+            raise ValueError("threads must be a multiple of 64 in [64, 1024]")
 
     @property
     def ndim(self) -> int:

@@ -28,7 +28,9 @@ class PrStaticDesc(ctypes.Structure):
 
 
 class PrLaunchDesc(ctypes.Structure):
-    _fields_ = [("fn", ctypes.c_void_p), ("tile", ctypes.c_int * 3), ("n_reduce", ctypes.c_int)]
+    _fields_ = [("fn", ctypes.c_void_p), ("tile", ctypes.c_int * 3), ("n_reduce", ctypes.c_int),
+                ("pack", ctypes.c_void_p), ("args_size", ctypes.c_size_t),
+                ("batched", ctypes.c_void_p)]
 
 
 class PrKernelDesc(ctypes.Structure):
@@ -48,7 +50,10 @@ RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_la
                    "pr_run", "pr_run_graph", "pr_run_timed", "pr_static_devptr", "pr_static_write",
                    "pr_static_read", "pr_static_bytes", "pr_size", "pr_cells",
                    "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load",
-                   "pr_adjoint_forward", "pr_adjoint_backward"]
+                   "pr_adjoint_forward", "pr_adjoint_backward",
+                   "pr_batch_create", "pr_batch_destroy", "pr_batch_grid", "pr_batch_run",
+                   "pr_batch_run_graph", "pr_batch_run_timed", "pr_batch_sync",
+                   "pr_batch_last_error", "pr_batch_cells"]
 
 _RT = None
 
@@ -82,6 +87,18 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_snapshot_alloc.argtypes = [vp, i, ctypes.POINTER(i), i]
     lib.pr_adjoint_forward.argtypes = [vp, i, i, i]
     lib.pr_adjoint_backward.argtypes = [vp, i, i, i, i]
+    lib.pr_batch_create.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(vp)]
+    lib.pr_batch_destroy.argtypes = [vp]
+    lib.pr_batch_grid.argtypes = [vp, i]
+    lib.pr_batch_grid.restype = vp
+    lib.pr_batch_run.argtypes = [vp, i, i]
+    lib.pr_batch_run_graph.argtypes = [vp, i, i]
+    lib.pr_batch_run_timed.argtypes = [vp, i, i, ctypes.POINTER(ctypes.c_double)]
+    lib.pr_batch_sync.argtypes = [vp]
+    lib.pr_batch_last_error.argtypes = [vp]
+    lib.pr_batch_last_error.restype = ctypes.c_char_p
+    lib.pr_batch_cells.argtypes = [vp]
+    lib.pr_batch_cells.restype = ctypes.c_longlong
     lib.pr_snapshot_save.argtypes = [vp, i]
     lib.pr_snapshot_load.argtypes = [vp, i]
     if path is None:
@@ -227,3 +244,81 @@ class Machine:
         self._check(self.rt.pr_adjoint_backward(self.h, self.kernel_index(fwd),
                                                 self.kernel_index(adj), nsteps, every),
                     "adjoint_backward")
+
+
+class _GridView(Machine):
+    """A grid of a Batch: a Machine that does not own its handle."""
+
+    def __init__(self, batch, g):   # noqa: D401 - no Machine.__init__
+        self.rt, self.lib, self.desc_ptr = batch.rt, batch.lib, batch.desc_ptr
+        self.name, self.ndim, self.statics, self.kernels = (batch.name, batch.ndim, batch.statics,
+                                                            batch.kernels)
+        self.shape = tuple(batch.sizes[g])
+        self.h = ctypes.c_void_p(batch.rt.pr_batch_grid(batch.b, g))
+        self._owner = batch
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many grids of one machine, of uneven sizes, advanced by the same launches."""
+
+    def __init__(self, machine_so: str, sizes):
+        if not os.path.exists(machine_so):
+            raise RuntimeError("machine library %s is not built" % machine_so)
+        self.rt = load_runtime()
+        self.lib = ctypes.CDLL(machine_so)
+        self.lib.pm_describe.restype = ctypes.POINTER(PrMachineDesc)
+        self.desc_ptr = self.lib.pm_describe()
+        d = self.desc_ptr.contents
+        self.name, self.ndim = d.name.decode(), d.ndim
+        self.statics = [(d.statics[s].name.decode(), d.statics[s].realm, PR_CT[d.statics[s].ctype])
+                        for s in range(d.n_static)]
+        self.kernels = [d.kernels[k].name.decode() for k in range(d.n_kernel)]
+        self.sizes = [tuple(int(x) for x in sz) for sz in sizes]
+        flat = []
+        for sz in self.sizes:
+            flat += list(sz) + [1] * (3 - len(sz))
+        arr = (ctypes.c_int * len(flat))(*flat)
+        b = ctypes.c_void_p()
+        rc = self.rt.pr_batch_create(ctypes.cast(self.desc_ptr, ctypes.c_void_p), len(self.sizes),
+                                     arr, ctypes.byref(b))
+        if rc != 0:
+            raise PrError("pr_batch_create failed (%d)" % rc)
+        self.b = b
+        self.grids = [_GridView(self, g) for g in range(len(self.sizes))]
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.rt.pr_batch_last_error(self.b)
+            raise PrError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def run(self, kernel, count=1, graph=False):
+        k = self.kernels.index(kernel) if isinstance(kernel, str) else kernel
+        fn = self.rt.pr_batch_run_graph if graph else self.rt.pr_batch_run
+        self._check(fn(self.b, k, count), "batch run %s" % kernel)
+
+    def run_timed(self, kernel, count=1):
+        k = self.kernels.index(kernel) if isinstance(kernel, str) else kernel
+        nl = self.desc_ptr.contents.kernels[k].n_launch
+        out = (ctypes.c_double * nl)()
+        self._check(self.rt.pr_batch_run_timed(self.b, k, count, out), "batch run_timed")
+        return list(out)
+
+    def sync(self):
+        self._check(self.rt.pr_batch_sync(self.b), "batch sync")
+
+    def cells(self) -> int:
+        return int(self.rt.pr_batch_cells(self.b))
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.rt.pr_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -57,11 +57,13 @@ def _wave_adj(ndim, real):
 # name -> (OM factory, setup, precise)
 VARIANTS = {
     "life_i32": (_life, Setup((64, 64), Language.HIP, boundary=CYC2), True),
-    "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), False),
+    "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2, batched=True),
+                  False),
     "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), True),
-    "hydro_f64": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    "hydro_f64": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2, batched=True),
+                  True),
     "hydro_adj_f32": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
-                                                     tile=(32, 8)), False),
+                                                     tile=(32, 8)), True),
     "hydro_adj_f64": (_hydro_adj(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                       tile=(32, 8)), True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
@@ -69,9 +71,27 @@ VARIANTS = {
 }
 
 
-def build(name: str, force: bool = False) -> str:
+def setup_of(name: str):
+    """The variant's setup, with the tile the autotuner recorded (paraiso_amd/tuned.json)."""
+    from .autotune import tuned
     mk, setup, precise = VARIANTS[name]
-    return build_machine(setup, mk(), name, precise=precise, force=force)
+    rec = tuned().get(name)
+    if rec is not None:
+        setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)))
+    return setup
+
+
+def build(name: str, force: bool = False) -> str:
+    mk, _, precise = VARIANTS[name]
+    return build_machine(setup_of(name), mk(), name, precise=precise, force=force)
+
+
+def build_all(force: bool = False, jobs: int = 4) -> dict:
+    """Build every variant; hipcc runs in ``jobs`` parallel processes."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = {n: ex.submit(build, n, force) for n in VARIANTS}
+        return {n: f.result() for n, f in futs.items()}
 
 
 def library(name: str) -> str:
@@ -83,3 +103,8 @@ def library(name: str) -> str:
 def machine(name: str, local_size):
     from .machine import Machine
     return Machine(library(name), local_size)
+
+
+def batch(name: str, sizes):
+    from .machine import Batch
+    return Batch(library(name), sizes)
