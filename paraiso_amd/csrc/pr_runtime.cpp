@@ -151,9 +151,11 @@ int pr_create(const pr_machine_desc* d, const int* local_size, pr_machine** out)
     m->n[a] = local_size[a];
     m->cells *= local_size[a];
   }
-  if (m->cells >= (1LL << 31)) {
+  if (m->cells >= (1LL << 31)) {   // generated kernels address cells with 32-bit offsets
+    int rc = fail(m, PR_E_INVALID, "grid of " + std::to_string(m->cells) +
+                                       " cells exceeds 2^31 - 1 cells");
     delete m;
-    return PR_E_INVALID;
+    return rc;
   }
   const int ns = d->n_static;
   m->cur.assign(ns, 0);

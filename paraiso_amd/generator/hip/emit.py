@@ -132,6 +132,8 @@ class MachineEmitter:
                  threads: int = NT):
         self.name = name
         self.nt = threads
+        self._reach = [[0, 0] for _ in range(3)]   # offsets seen by _wrap in the current phase
+        self._direct = False
         self.fast_fp32 = fast_fp32
         self.batched = batched
         self.statics = statics           # list of Named
@@ -368,36 +370,62 @@ class MachineEmitter:
         w("  // phase %d: %s over box %s" % (ph.index, "store" if is_out else
                                              "values " + ",".join("n%d" % v for v in ph.nodes),
                                              list(box[:nd])))
-        w("  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, self.nt))
-        w("    const int l0 = p %% %d;" % W[0])
-        w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
-        w("    const int l2 = p / %d;" % (W[0] * W[1]))
-        w("    (void)l0; (void)l1; (void)l2;")
+        roots = [t for t in la.targets if t not in la.mat] if is_out else list(ph.nodes)
+        self._reach = [[0, 0] for _ in range(3)]
+        self._direct = False
+        wrapped = self._ctx_body(k, la, roots, box, W, is_out)
+        reach = self._reach
+        if any("om_wrap(" in ln for ln in wrapped):
+            # tile-uniform test: every wrapped coordinate of this phase lies inside the grid,
+            # so interior tiles index directly and only tiles on the border wrap
+            self._direct = True
+            direct = self._ctx_body(k, la, roots, box, W, is_out)
+            self._direct = False
+            conds = []
+            for ax in range(nd):
+                lo = box[ax][0] + reach[ax][0]
+                hi = T[ax] + box[ax][1] - 1 + reach[ax][1]
+                conds.append("X%d + (%d) >= 0 && X%d + (%d) < a.n%d" % (ax, lo, ax, hi, ax))
+            w("  if (%s) {" % " && ".join(conds))
+            self._emit_loop(k, la, ph, box, W, vol, is_out, roots, direct, w, "  ")
+            w("  } else {")
+            self._emit_loop(k, la, ph, box, W, vol, is_out, roots, wrapped, w, "  ")
+            w("  }")
+        else:
+            self._emit_loop(k, la, ph, box, W, vol, is_out, roots, wrapped, w, "")
+        w("  __syncthreads();")
+
+    def _emit_loop(self, k, la, ph, box, W, vol, is_out, roots, lines, w, ind):
+        nodes = k.nodes
+        nd = self.ndim
+        w(ind + "  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, self.nt))
+        w(ind + "    const int l0 = p %% %d;" % W[0])
+        w(ind + "    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
+        w(ind + "    const int l2 = p / %d;" % (W[0] * W[1]))
+        w(ind + "    (void)l0; (void)l1; (void)l2;")
         for ax in range(3):
-            w("    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+            w(ind + "    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
         if is_out:
             conds = ["g%d < a.n%d" % (ax, ax) for ax in range(nd)]
-            w("    if (!(%s)) continue;" % " && ".join(conds))
-            w("    const int addr0 = %s;" % self._addr(["g0", "g1", "g2"]))
-        roots = [t for t in la.targets if t not in la.mat] if is_out else list(ph.nodes)
-        ctx = OUT if is_out else None
-        lines = self._ctx_body(k, la, roots, box, W, is_out)
+            w(ind + "    if (!(%s)) continue;" % " && ".join(conds))
+            w(ind + "    const int addr0 = %s;" % self._addr(["g0", "g1", "g2"]))
         for ln in lines:
-            w("    " + ln)
+            w(ind + "    " + ln)
         zero = tuple(0 for _ in range(nd))
         if is_out:
             for s, v in la.stores:
-                w("    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s), self._val_name(k, la, v, zero, roots)))
+                w(ind + "    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s),
+                                                    self._val_name(k, la, v, zero, roots)))
             for r in la.reduces:
                 arg = nodes[r].args[0]
                 at = acc_type(nodes[r].payload, nodes[arg].ctype)
-                w("    acc%d = %s(acc%d, (%s)%s);" % (r, acc_op(nodes[r].payload, at), r, at.c_name,
-                                                     self._val_name(k, la, arg, zero, roots)))
+                w(ind + "    acc%d = %s(acc%d, (%s)%s);" % (
+                    r, acc_op(nodes[r].payload, at), r, at.c_name,
+                    self._val_name(k, la, arg, zero, roots)))
         else:
             for v in ph.nodes:
-                w("    s%d[p] = n%d_%s;" % (v, v, _cname(zero)))
-        w("  }")
-        w("  __syncthreads();")
+                w(ind + "    s%d[p] = n%d_%s;" % (v, v, _cname(zero)))
+        w(ind + "  }")
 
     def _val_name(self, k, la, v, c, roots):
         return "n%d_%s" % (v, _cname(c))
@@ -458,8 +486,7 @@ class MachineEmitter:
                     lines.append("const %s %s = %s;" % (tn, name, c_literal(n.payload, n.ctype)))
                 elif n.op == "index":
                     ax = n.payload
-                    lines.append("const %s %s = (%s)om_wrap(g%d + %d, a.n%d);" % (tn, name, tn, ax,
-                                                                                c[ax], ax))
+                    lines.append("const %s %s = (%s)%s;" % (tn, name, tn, self._wrap(ax, c[ax])))
                 elif n.op == "size":
                     lines.append("const %s %s = (%s)a.n%d;" % (tn, name, tn, n.payload))
                 elif n.op == "bcast":
@@ -480,11 +507,17 @@ class MachineEmitter:
         nd = self.ndim
         if is_out and all(x == 0 for x in c):
             return "addr0"
-        g = []
-        for ax in range(nd):
-            g.append("om_wrap(g%d + %d, a.n%d)" % (ax, c[ax], ax))
+        g = [self._wrap(ax, c[ax]) for ax in range(nd)]
         g += ["0"] * (3 - nd)
         return self._addr(g)
+
+    def _wrap(self, ax: int, off: int) -> str:
+        """Coordinate g<ax> + off on the periodic grid (direct inside interior tiles)."""
+        r = self._reach[ax]
+        r[0], r[1] = min(r[0], off), max(r[1], off)
+        if self._direct:
+            return "(g%d + %d)" % (ax, off)
+        return "om_wrap(g%d + %d, a.n%d)" % (ax, off, ax)
 
     # ------------------------------------------------------------ machine
     def emit(self) -> str:
