@@ -158,8 +158,11 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
             "grad_density_l1": float(np.abs(g).sum())}
 
 
-def bench_wave(n=512, steps=300):
-    """LinearWave 3-D, 512^3 FP32, c = 1, dt = 0.5 dx, Gaussian pulse (8 cells)."""
+def bench_wave(n=512, steps=300, every=20):
+    """LinearWave 3-D, 512^3 FP32, c = 1, dt = 0.5 dx, Gaussian pulse (8 cells, seed 3).
+
+    Then forward + checkpointed adjoint of the misfit J = 1/2 sum (f_N - d)^2 with
+    d the same pulse from seed 4; the adjoint is seeded with f_N - d."""
     import numpy as np
     import torch
     from paraiso_amd import inputs, variants
@@ -180,15 +183,62 @@ def bench_wave(n=512, steps=300):
     cells = n ** 3
     dom = max(range(len(ms)), key=lambda l: ms[l])
     achieved = cells * 16 / (ms[dom] / steps * 1e-3) / 1e9
-    return {"workload": "wave3d-%d^3-fp32" % n, "steps": steps,
-            "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
-            "ms_per_step": (t1 - t0) / steps * 1e3, "achieved_GBps": achieved,
-            "frac": achieved / HBM_PEAK_GBPS, "algorithmic_bytes_per_cell": 16,
-            "energy_drift": abs(m.get("energy") - e0) / abs(e0)}
+    out = {"workload": "wave3d-%d^3-fp32" % n, "steps": steps,
+           "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
+           "ms_per_step": (t1 - t0) / steps * 1e3, "achieved_GBps": achieved,
+           "frac": achieved / HBM_PEAK_GBPS, "algorithmic_bytes_per_cell": 16,
+           "energy_drift": abs(m.get("energy") - e0) / abs(e0)}
+    # forward + adjoint of the misfit
+    pulse = inputs.gaussian_pulse((n, n, n), 3, dtype=np.float32)
+    target = inputs.gaussian_pulse((n, n, n), 4, dtype=np.float32)
+    S = (steps + every - 1) // every
+    names = [x[0] for x in m.statics]
+    m.snapshot_alloc(S + every, [x for x in names if not x.startswith(("adj_", "__"))])
+
+    def reset():
+        m.set("fieldF", pulse)
+        m.set("fieldG", 0.0)
+        m.sync()
+    reset()
+    t0 = time.perf_counter()
+    m.run("proceed", steps)
+    m.sync()
+    t_fwd = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    m.adjoint_forward("proceed", steps, every)
+    m.sync()
+    t_af = time.perf_counter() - t0
+    r = m.get("fieldF").astype(np.float64) - target
+    misfit = 0.5 * float((r * r).sum())
+    m.set("adj_fieldF", r.astype(np.float32))
+    m.set("adj_fieldG", 0.0)
+    m.sync()
+    t0 = time.perf_counter()
+    m.adjoint_backward("proceed", "proceed_adjoint", steps, every)
+    m.sync()
+    t_ab = time.perf_counter() - t0
+    out["adjoint"] = {"misfit": misfit, "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
+                      "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
+                      "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
+                      "grad_l2": float(np.sqrt((m.get("adj_fieldF").astype(np.float64) ** 2).sum()))}
+    return out
 
 
-def cpu_baseline(nx, ny, budget_s=12.0, max_steps=60):
-    """Oracle CPU machine (PlanTrans restatement, OpenMP), bounded sample."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nx, ny, runs=3, budget_s=5.0, max_steps=40):
+    """Oracle CPU machine (PlanTrans restatement, OpenMP on every host thread the process
+    may use), bounded sample: `runs` timed runs of about `budget_s` each, median."""
     import numpy as np
     from fractions import Fraction
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -196,18 +246,24 @@ def cpu_baseline(nx, ny, budget_s=12.0, max_steps=60):
     from paraiso_amd.om.dyn import CType
     from paraiso_amd.schemes.hydro import hydro_om
     m = oracle_machine(lambda: hydro_om(CType.Float, Fraction(7, 5)), (nx, ny))
-    setup_hydro(m, nx, ny, 1, np.float32)
-    m.run("proceed", 1)
-    steps, t0 = 0, time.perf_counter()
-    while steps < max_steps and time.perf_counter() - t0 < budget_s:
-        m.run("proceed", 1)
-        steps += 1
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nx * ny * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d steps of Hydro %dx%d FP32 (%.1f s), OpenMP, generated by the oracle's "
-                      "restatement of the reference's CPlusPlus emitter" % (steps, nx, ny, dt)}
+    rates, total_steps, total_s = [], 0, 0.0
+    for r in range(runs):
+        setup_hydro(m, nx, ny, 1, np.float32)
+        m.run("proceed", 1)                        # first touch outside the timed region
+        steps, t0 = 0, time.perf_counter()
+        while steps < max_steps and time.perf_counter() - t0 < budget_s:
+            m.run("proceed", 1)
+            steps += 1
+        dt = time.perf_counter() - t0
+        rates.append(nx * ny * steps / dt / 1e9)
+        total_steps += steps
+        total_s += dt
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {"value": float(np.median(rates)), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port", "runs": [round(x, 5) for x in rates], "cpu_model": _cpu_model(),
+            "sample": "median of %d runs, %d steps of Hydro %dx%d in %.1f s, generated by the "
+                      "oracle's restatement of the reference's CPlusPlus emitter (OpenMP, "
+                      "g++ -O2 -ffp-contract=off)" % (runs, total_steps, nx, ny, total_s)}
 
 
 def main():

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PR_ABI_VERSION 2
+#define PR_ABI_VERSION 3
 
 enum pr_status {
   PR_OK = 0,
@@ -109,6 +109,8 @@ typedef struct pr_machine_desc {
   const pr_kernel_desc* kernels;
   int n_slot;              /* scalar slots after the statics in the scalar block */
   int halo;                /* minimum grid extent per axis the kernels need */
+  int lower_margin[3];     /* ghost cells below / above the local size on Open axes */
+  int upper_margin[3];     /* (Plan.lowerMargin / upperMargin, Generator/Plan.hs:33-92) */
 } pr_machine_desc;
 
 /* Exported by every generated machine library. */
@@ -139,6 +141,9 @@ size_t pr_static_bytes(const pr_machine* m, int s);
 
 /* ---- sizes (replace om_size, om_memory_size and friends, PlanTrans.hs:167-215) */
 int pr_size(const pr_machine* m, int axis);
+/* Extent of the arrays, margins included (om_memory_size_k, PlanTrans.hs:167-215); equals
+ * pr_size on Cyclic axes.  Static arrays hold prod(pr_memory_size) elements. */
+int pr_memory_size(const pr_machine* m, int axis);
 long long pr_cells(const pr_machine* m);
 
 /* ---- batches: many independent grids of one machine, of uneven sizes,

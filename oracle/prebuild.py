@@ -26,6 +26,9 @@ def _cases():
     for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96)]:
         out.append((life_om, n, cyc2))
     out.append((lambda: hydro_om(CType.Float, G), (2048, 2048), cyc2))   # bench CPU baseline
+    open2 = (A.Condition.Open,) * 2
+    for n in [(64, 64), (100, 37), (9, 130)]:
+        out.append((life_om, n, open2))
     from paraiso_amd.adjoint import add_adjoint
     from paraiso_amd.schemes.linear_wave import linear_wave_om
     cyc3 = (A.Condition.Cyclic,) * 3

@@ -3,6 +3,7 @@
 // The generator instantiates these for each kernel of an Orthotope Machine:
 //   * om_wrap        cyclic index wrap for halo reads
 //                    (the reference's `(x + c + n) % n`, Generator/PlanTrans.hs:459-461,481-483),
+//   * om_clamp       keeps halo reads of Open (non-periodic) axes inside the grid,
 //   * om_max/om_min  exactly std::max/std::min (the reference's CPU Max/Min,
 //                    Generator/PlanTrans.hs:690-691): ties and NaNs resolve as there,
 //   * om_block_*     wave64 shuffle tree + LDS combine for Reduce: each
@@ -26,6 +27,8 @@ __device__ __forceinline__ int om_wrap(int x, int n) {
   if ((unsigned)x >= (unsigned)n) x = ((x % n) + n) % n;
   return x;
 }
+
+__device__ __forceinline__ int om_clamp(int x, int n) { return x < 0 ? 0 : (x < n ? x : n - 1); }
 
 template <class T> __device__ __forceinline__ T om_max(T a, T b) { return (a < b) ? b : a; }
 template <class T> __device__ __forceinline__ T om_min(T a, T b) { return (b < a) ? b : a; }

@@ -36,8 +36,9 @@ struct GraphKey {
 
 struct pr_machine {
   const pr_machine_desc* d = nullptr;
-  int n[3] = {1, 1, 1};
-  long long cells = 1;
+  int n[3] = {1, 1, 1};              // array extents (local size plus margins)
+  int local[3] = {1, 1, 1};          // local size (the reference's localSize)
+  long long cells = 1;               // cells of one static array
   std::vector<void*> buf[2];        // per static
   std::vector<int> cur;             // parity per static
   std::vector<char> stored_any;     // static stored by some kernel
@@ -148,8 +149,9 @@ int pr_create(const pr_machine_desc* d, const int* local_size, pr_machine** out)
       delete m;
       return rc;
     }
-    m->n[a] = local_size[a];
-    m->cells *= local_size[a];
+    m->local[a] = local_size[a];
+    m->n[a] = local_size[a] + d->lower_margin[a] + d->upper_margin[a];
+    m->cells *= m->n[a];
   }
   if (m->cells >= (1LL << 31)) {   // generated kernels address cells with 32-bit offsets
     int rc = fail(m, PR_E_INVALID, "grid of " + std::to_string(m->cells) +
@@ -368,6 +370,11 @@ int pr_static_read(pr_machine* m, int s, void* dst, size_t bytes, int dst_on_dev
 }
 
 int pr_size(const pr_machine* m, int axis) {
+  if (!m || axis < 0 || axis >= 3) return 0;
+  return m->local[axis];
+}
+
+int pr_memory_size(const pr_machine* m, int axis) {
   if (!m || axis < 0 || axis >= 3) return 0;
   return m->n[axis];
 }

@@ -15,7 +15,7 @@ import subprocess
 from typing import List, Optional, Tuple
 
 from .. import annotation as A
-from ..om.graph import OM
+from ..om.graph import OM, Reduce, Store
 from .native import Language, Setup, default_setup
 from .plan import Plan, translate
 
@@ -38,21 +38,76 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
     from .hip.host import emit_host_class
     from .hip.ir import lower_kernel
     from .hip.schedule import lds_budget, schedule_kernel
-    if any(b is not A.Condition.Cyclic for b in setup.boundary):
-        raise NotImplementedError("the HIP backend lowers Cyclic boundaries; Open margins are not "
-                                  "implemented for the GPU path")
     plan = translate(setup, om)
     om_opt = plan.om
     tile = tuple(setup.tile) if setup.tile else DEFAULT_TILES[om_opt.ndim]
     cyclic = tuple(b is A.Condition.Cyclic for b in setup.boundary)
     budget = lds_budget(setup.threads)
-    scheds = [schedule_kernel(lower_kernel(om_opt, ki, cyclic), tile, budget)
-              for ki in range(len(om_opt.kernels))]
+    lowered = [lower_kernel(om_opt, ki, cyclic) for ki in range(len(om_opt.kernels))]
+    regions = _target_regions(plan, lowered) if not all(cyclic) else {}
+    scheds = [schedule_kernel(k, tile, budget) for k in lowered]
     src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,
                          fast_fp32=not precise, batched=setup.batched,
-                         threads=setup.threads).emit()
-    hpp = emit_host_class(om_opt, setup)
+                         threads=setup.threads, open_axes=tuple(not c for c in cyclic),
+                         margins=(plan.lower_margin, plan.upper_margin),
+                         regions=regions).emit()
+    hpp = emit_host_class(om_opt, setup, (plan.lower_margin, plan.upper_margin))
     return plan, src, hpp
+
+
+def _target_regions(plan: Plan, lowered) -> dict:
+    """Memory region [lb, n - ub) over which each store and reduce argument is computed.
+
+    The reference computes a Manifest array only inside its subkernel's bounds, and the
+    rest of its storage keeps the zeros it was allocated with (``PlanTrans.hs:75-86``);
+    a Store copies the whole storage into the static (``PlanTrans.hs:243-258``) and a
+    Reduce folds the whole storage (``PlanTrans.hs:719``).  Its loop addresses cells
+    from the plan's lower margin whenever the bounds differ from the memory size
+    (``PlanTrans.hs:440-454``), so bounds that neither cover the memory nor start at
+    that margin read and write outside the arrays; such programs are rejected."""
+    from .plan import _valid_to_lower, _valid_to_upper
+    setup, om = plan.setup, plan.om
+    lm, um = plan.lower_margin, plan.upper_margin
+    out = {}
+    for sub in plan.subkernels:
+        lb, ub = tuple(sub.lower_boundary), tuple(sub.upper_boundary)
+        if any(lb + ub) and lb != tuple(lm):
+            raise NotImplementedError(
+                "subkernel %s of %s computes over [%s, n - %s), which the reference's addressing "
+                "(origin at the plan margin %s) does not keep inside its arrays"
+                % (sub.name, plan.name, list(lb), list(ub), list(lm)))
+
+    def region(g, value):
+        valids = A.to_list(A.Valid, g.nodes[value].anot)
+        if not valids:
+            return None
+        lb, ub = _valid_to_lower(valids[0], setup), _valid_to_upper(valids[0], setup)
+        full = all(x == 0 for x in lb + ub)
+        if not full and tuple(lb) != tuple(lm):
+            raise NotImplementedError(
+                "a subkernel of %s computes over [%s, n - %s), which the reference's addressing "
+                "(origin at the plan margin %s) does not keep inside its arrays"
+                % (plan.name, list(lb), list(ub), list(lm)))
+        return tuple(lb), tuple(ub)
+
+    for ki, (k, lk) in enumerate(zip(om.kernels, lowered)):
+        g = k.graph
+        for v, node in enumerate(g.nodes):
+            if isinstance(node.inst, Store):
+                r = region(g, g.pre(v)[0])
+                if r is not None:
+                    out[(ki, "store", node.inst.static)] = r
+        for n in lk.nodes:
+            if n.op != "reduce":
+                continue
+            for ov in n.om_nodes:
+                inst_id, inst = g.inst_of_value(ov)
+                if isinstance(inst, Reduce):
+                    r = region(g, g.pre(inst_id)[0])
+                    if r is not None:
+                        out[(ki, "reduce", n.id)] = r
+                    break
+    return out
 
 
 def generate(setup: Setup, om: OM) -> List[Tuple[str, str]]:

@@ -129,9 +129,17 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
 class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
                  tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
-                 threads: int = NT):
+                 threads: int = NT, open_axes: Tuple[bool, ...] = (), margins=None,
+                 regions=None):
         self.name = name
         self.nt = threads
+        # Open (non-periodic) axes: grids carry lower/upper margins of ghost cells; a store or
+        # reduce argument computed over region [lb, n - ub) holds 0 elsewhere
+        self.open_axes = tuple(open_axes) + (False,) * (3 - len(open_axes))
+        lm, um = margins if margins else ((0,) * ndim, (0,) * ndim)
+        self.lm = tuple(lm) + (0,) * (3 - len(lm))
+        self.um = tuple(um) + (0,) * (3 - len(um))
+        self.regions = regions or {}     # (kernel, "store", static) / (kernel, "reduce", node)
         self._reach = [[0, 0] for _ in range(3)]   # offsets seen by _wrap in the current phase
         self._direct = False
         self.fast_fp32 = fast_fp32
@@ -176,7 +184,7 @@ class MachineEmitter:
         if n.op == "imm":
             return c_literal(n.payload, n.ctype)
         if n.op == "size":
-            return "(%s)(a.n%d)" % (n.ctype.c_name, n.payload)
+            return "(%s)%s" % (n.ctype.c_name, self._local_size(n.payload))
         if n.op == "arith":
             op, cast_to = n.payload
             return device_arith(op, ["u%d" % x for x in n.args], cast_to,
@@ -414,14 +422,19 @@ class MachineEmitter:
         zero = tuple(0 for _ in range(nd))
         if is_out:
             for s, v in la.stores:
-                w(ind + "    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s),
-                                                    self._val_name(k, la, v, zero, roots)))
+                val = self._val_name(k, la, v, zero, roots)
+                cond = self._region_cond((self._ki, "store", s))
+                if cond:
+                    val = "(%s) ? %s : (%s)0" % (cond, val, nodes[v].ctype.c_name)
+                w(ind + "    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s), val))
             for r in la.reduces:
                 arg = nodes[r].args[0]
                 at = acc_type(nodes[r].payload, nodes[arg].ctype)
-                w(ind + "    acc%d = %s(acc%d, (%s)%s);" % (
-                    r, acc_op(nodes[r].payload, at), r, at.c_name,
-                    self._val_name(k, la, arg, zero, roots)))
+                val = "(%s)%s" % (at.c_name, self._val_name(k, la, arg, zero, roots))
+                cond = self._region_cond((self._ki, "reduce", r))
+                if cond:
+                    val = "(%s) ? %s : (%s)0" % (cond, val, at.c_name)
+                w(ind + "    acc%d = %s(acc%d, %s);" % (r, acc_op(nodes[r].payload, at), r, val))
         else:
             for v in ph.nodes:
                 w(ind + "    s%d[p] = n%d_%s;" % (v, v, _cname(zero)))
@@ -486,9 +499,14 @@ class MachineEmitter:
                     lines.append("const %s %s = %s;" % (tn, name, c_literal(n.payload, n.ctype)))
                 elif n.op == "index":
                     ax = n.payload
-                    lines.append("const %s %s = (%s)%s;" % (tn, name, tn, self._wrap(ax, c[ax])))
+                    if self.open_axes[ax]:       # the cell's coordinate inside the margins
+                        idx = "(g%d + %d)" % (ax, c[ax] - self.lm[ax])
+                    else:
+                        idx = self._wrap(ax, c[ax])
+                    lines.append("const %s %s = (%s)%s;" % (tn, name, tn, idx))
                 elif n.op == "size":
-                    lines.append("const %s %s = (%s)a.n%d;" % (tn, name, tn, n.payload))
+                    lines.append("const %s %s = (%s)%s;" % (tn, name, tn,
+                                                           self._local_size(n.payload)))
                 elif n.op == "bcast":
                     lines.append("const %s %s = u%d;" % (tn, name, n.args[0]))
                 elif n.op == "shift":
@@ -517,7 +535,26 @@ class MachineEmitter:
         r[0], r[1] = min(r[0], off), max(r[1], off)
         if self._direct:
             return "(g%d + %d)" % (ax, off)
+        if self.open_axes[ax]:
+            # only cells outside every store's region read past the grid; clamping keeps
+            # those reads in bounds and their results are never stored
+            return "om_clamp(g%d + %d, a.n%d)" % (ax, off, ax)
         return "om_wrap(g%d + %d, a.n%d)" % (ax, off, ax)
+
+    def _local_size(self, ax: int) -> str:
+        m = self.lm[ax] + self.um[ax]
+        return "(a.n%d - %d)" % (ax, m) if m else "(a.n%d)" % ax
+
+    def _region_cond(self, key) -> Optional[str]:
+        """C condition that (g0, g1, g2) lies in the target's region, or None if everywhere."""
+        lb, ub = self.regions.get(key, ((0,) * self.ndim, (0,) * self.ndim))
+        conds = []
+        for ax in range(self.ndim):
+            if lb[ax]:
+                conds.append("g%d >= %d" % (ax, lb[ax]))
+            if ub[ax]:
+                conds.append("g%d < a.n%d - %d" % (ax, ax, ub[ax]))
+        return " && ".join(conds) if conds else None
 
     # ------------------------------------------------------------ machine
     def emit(self) -> str:
@@ -565,7 +602,8 @@ class MachineEmitter:
         out.append("static const pr_static_desc k_statics[] = {%s};" % ", ".join(sdesc))
         out.append("static const pr_kernel_desc k_kernels[] = {%s};" % ", ".join(kdescs))
         out.append('static const pr_machine_desc k_desc = {PR_ABI_VERSION, "%s", %d, %d, k_statics, '
-                   '%d, k_kernels, %d, %d};' % (self.name, self.ndim, self.ns, len(self.schedules),
-                                                len(self.slot_index), 1))
+                   '%d, k_kernels, %d, %d, {%d, %d, %d}, {%d, %d, %d}};'
+                   % ((self.name, self.ndim, self.ns, len(self.schedules), len(self.slot_index), 1)
+                      + self.lm + self.um))
         out.append('extern "C" const pr_machine_desc* pm_describe(void) { return &k_desc; }')
         return "\n".join(out) + "\n"

@@ -14,7 +14,7 @@ from __future__ import annotations
 from ...om.dyn import Realm
 
 
-def emit_host_class(om, setup) -> str:
+def emit_host_class(om, setup, margins=None) -> str:
     name = om.name
     L = []
     w = L.append
@@ -46,16 +46,18 @@ def emit_host_class(om, setup) -> str:
     w("  %s(const %s&) = delete;" % (name, name))
     w("  %s& operator=(const %s&) = delete;" % (name, name))
     w("  pr_machine* handle() { return m_; }")
-    total = 1
-    for x in setup.local_size:
+    lm, um = margins if margins else ((0,) * setup.ndim, (0,) * setup.ndim)
+    total = mem_total = 1
+    for ax, x in enumerate(setup.local_size):
         total *= x
+        mem_total *= x + lm[ax] + um[ax]
     w("  int om_size() const { return %d; }" % total)
-    w("  int om_memory_size() const { return %d; }" % total)
+    w("  int om_memory_size() const { return %d; }" % mem_total)
     for ax, x in enumerate(setup.local_size):
         w("  int om_size_%d() const { return %d; }" % (ax, x))
-        w("  int om_memory_size_%d() const { return %d; }" % (ax, x))
-        w("  int om_lower_margin_%d() const { return 0; }" % ax)
-        w("  int om_upper_margin_%d() const { return 0; }" % ax)
+        w("  int om_memory_size_%d() const { return %d; }" % (ax, x + lm[ax] + um[ax]))
+        w("  int om_lower_margin_%d() const { return %d; }" % (ax, lm[ax]))
+        w("  int om_upper_margin_%d() const { return %d; }" % (ax, um[ax]))
     for ki, k in enumerate(om.kernels):
         w("  void %s() { check(pr_run(m_, %d, 1), m_, \"%s\"); }" % (k.name, ki, k.name))
     for si, s in enumerate(om.setup.static_values):
@@ -66,7 +68,7 @@ def emit_host_class(om, setup) -> str:
             w("  void set_%s(%s v) { check(pr_static_write(m_, %d, &v, sizeof v, 0), m_, \"write\"); }"
               % (s.name, t, si))
         else:
-            w("  std::vector<%s> %s() { std::vector<%s> v(om_size()); check(pr_static_read(m_, %d, v.data(), "
+            w("  std::vector<%s> %s() { std::vector<%s> v(om_memory_size()); check(pr_static_read(m_, %d, v.data(), "
               "v.size() * sizeof(%s), 0), m_, \"read\"); return v; }" % (t, s.name, t, si, t))
             w("  void set_%s(const std::vector<%s>& v) { check(pr_static_write(m_, %d, v.data(), "
               "v.size() * sizeof(%s), 0), m_, \"write\"); }" % (s.name, t, si, t))

@@ -43,12 +43,13 @@ class PrMachineDesc(ctypes.Structure):
     _fields_ = [("abi", ctypes.c_int), ("name", ctypes.c_char_p), ("ndim", ctypes.c_int),
                 ("n_static", ctypes.c_int), ("statics", ctypes.POINTER(PrStaticDesc)),
                 ("n_kernel", ctypes.c_int), ("kernels", ctypes.POINTER(PrKernelDesc)),
-                ("n_slot", ctypes.c_int), ("halo", ctypes.c_int)]
+                ("n_slot", ctypes.c_int), ("halo", ctypes.c_int),
+                ("lower_margin", ctypes.c_int * 3), ("upper_margin", ctypes.c_int * 3)]
 
 
 RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_last_error",
                    "pr_run", "pr_run_graph", "pr_run_timed", "pr_static_devptr", "pr_static_write",
-                   "pr_static_read", "pr_static_bytes", "pr_size", "pr_cells",
+                   "pr_static_read", "pr_static_bytes", "pr_size", "pr_memory_size", "pr_cells",
                    "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load",
                    "pr_adjoint_forward", "pr_adjoint_backward",
                    "pr_batch_create", "pr_batch_destroy", "pr_batch_grid", "pr_batch_run",
@@ -82,6 +83,7 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_static_bytes.argtypes = [vp, i]
     lib.pr_static_bytes.restype = sz
     lib.pr_size.argtypes = [vp, i]
+    lib.pr_memory_size.argtypes = [vp, i]
     lib.pr_cells.argtypes = [vp]
     lib.pr_cells.restype = ctypes.c_longlong
     lib.pr_snapshot_alloc.argtypes = [vp, i, ctypes.POINTER(i), i]
@@ -127,7 +129,12 @@ class Machine:
             self.statics.append((sd.name.decode(), sd.realm, PR_CT[sd.ctype]))
         self.kernels = [d.kernels[k].name.decode() for k in range(d.n_kernel)]
         n = list(local_size) + [1] * (3 - len(local_size))
-        self.shape = tuple(local_size)
+        self.local_shape = tuple(local_size)
+        # arrays carry the margins of Open axes (om_memory_size_k)
+        self.shape = tuple(int(x) + d.lower_margin[a] + d.upper_margin[a]
+                           for a, x in enumerate(local_size))
+        self.lower_margin = tuple(d.lower_margin[a] for a in range(self.ndim))
+        self.upper_margin = tuple(d.upper_margin[a] for a in range(self.ndim))
         arr = (ctypes.c_int * 3)(*n)
         h = ctypes.c_void_p()
         rc = self.rt.pr_create(ctypes.cast(self.desc_ptr, ctypes.c_void_p), arr, ctypes.byref(h))
@@ -253,8 +260,9 @@ class _GridView(Machine):
         self.rt, self.lib, self.desc_ptr = batch.rt, batch.lib, batch.desc_ptr
         self.name, self.ndim, self.statics, self.kernels = (batch.name, batch.ndim, batch.statics,
                                                             batch.kernels)
-        self.shape = tuple(batch.sizes[g])
         self.h = ctypes.c_void_p(batch.rt.pr_batch_grid(batch.b, g))
+        self.local_shape = tuple(batch.sizes[g])
+        self.shape = tuple(self.rt.pr_memory_size(self.h, a) for a in range(self.ndim))
         self._owner = batch
 
     def close(self):

@@ -14,11 +14,22 @@ from .om.dyn import CType
 
 CYC2 = (A.Condition.Cyclic,) * 2
 CYC3 = (A.Condition.Cyclic,) * 3
+OPEN2 = (A.Condition.Open,) * 2
 
 
 def _life():
     from .schemes.life import life_om
     return life_om()
+
+
+def _life_legacy():
+    from .schemes.life import life_om_legacy
+    return life_om_legacy()
+
+
+def _hydro_legacy():
+    from .schemes.hydro import hydro_om_legacy
+    return hydro_om_legacy()
 
 
 def _hydro(real, gamma=Fraction(7, 5), dt_mode="cfl"):
@@ -68,6 +79,11 @@ VARIANTS = {
                                                       tile=(32, 8)), True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
     "wave3d_f64": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP, boundary=CYC3), True),
+    # Open (non-periodic) boundaries: ghost margins, as the reference's shipped examples
+    "life_open_i32": (_life, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
+    "life_legacy_open_i32": (_life_legacy, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
+    "hydro_legacy_open_f32": (_hydro_legacy, Setup((1024, 1024), Language.HIP, boundary=OPEN2),
+                              True),
 }
 
 

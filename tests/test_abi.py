@@ -42,7 +42,7 @@ def test_machine_library_describes_its_program(name):
     lib = ctypes.CDLL(so)
     lib.pm_describe.restype = ctypes.POINTER(PrMachineDesc)
     d = lib.pm_describe().contents
-    assert d.abi == 2
+    assert d.abi == 3
     mk, setup, _ = variants.VARIANTS[name]
     om = mk()
     user = [d.statics[i].name.decode() for i in range(d.n_static) if not d.statics[i].flags & 1]

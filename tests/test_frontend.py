@@ -165,3 +165,18 @@ def test_reference_grouping_can_precede_a_dependency():
 def test_optimize_skips_a_recorded_level():
     om = optimize(A.Level.O3, life_om())
     assert optimize(A.Level.O1, om) is om
+
+
+def test_open_boundary_generation():
+    """Open axes lower with ghost margins; a program whose reference code would address
+    cells outside its arrays (current Hydro: a subkernel bounded at [1, n) while the
+    loop origin is the plan margin 3, PlanTrans.hs:440-454) is rejected."""
+    from paraiso_amd.generator import _hip_sources
+    from paraiso_amd.schemes.life import life_om_legacy
+    open2 = (A.Condition.Open,) * 2
+    plan, src, hpp = _hip_sources(Setup((32, 32), Language.HIP, boundary=open2), life_om_legacy())
+    assert plan.lower_margin == (1, 1) and "om_clamp" in src
+    assert "{1, 1, 0}, {1, 1, 0}" in src                       # margins in the descriptor
+    assert "om_memory_size() const { return 1156; }" in hpp     # 34 * 34
+    with pytest.raises(NotImplementedError):
+        _hip_sources(Setup((32, 32), Language.HIP, boundary=open2), hydro_om(CType.Double))
