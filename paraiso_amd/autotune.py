@@ -92,7 +92,9 @@ def _setup_inputs(variant: str, m, size):
 
 
 def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=None,
-            kernel: str = "proceed") -> List[Dict]:
+            kernel: str = None) -> List[Dict]:
+    """Device time per step of every compiled candidate; the step is ``proceed_pipelined``
+    (after ``proceed_prime``) where the program has it, else ``kernel`` or ``proceed``."""
     from . import variants
     from .generator import GEN_DIR
     from .machine import Machine
@@ -107,10 +109,16 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
         print("tune %s tile=%s threads=%d" % (variant, tile, nt), flush=True)
         m = Machine(so, size)
         _setup_inputs(variant, m, size)
-        m.run(kernel, warmup)
+        step = kernel
+        if step is None:
+            step = "proceed_pipelined" if "proceed_pipelined" in m.kernels else "proceed"
+            if step == "proceed_pipelined":
+                m.run("proceed_prime")
+        m.run(step, warmup)
         m.sync()
-        ms = m.run_timed(kernel, steps)
-        results.append({"tile": list(tile), "threads": nt, "ms_per_step": sum(ms) / steps,
+        ms = m.run_timed(step, steps)
+        results.append({"tile": list(tile), "threads": nt, "kernel": step,
+                        "ms_per_step": sum(ms) / steps,
                         "launch_ms": [x / steps for x in ms]})
         m.close()
     return results

@@ -31,6 +31,7 @@ from ...om.dyn import CType, Realm
 from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
+SCHED_DEBUG = False
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
 LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches are cut
 LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
@@ -394,30 +395,68 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                 top = max(top, off + nbytes)
         return phase, out_phase, members, lc, phases, lds, top
 
-    for _round in range(64):
-        phase, out_phase, members, lc = analyse(M)
-        lcost = local_costs(M)
-        ranked = []
-        for v in computed:
-            if v in M or nodes[v].op != "arith":
+    def greedy(M, limit, rounds):
+        for _round in range(rounds):
+            phase, out_phase, members, lc = analyse(M)
+            lcost = local_costs(M)
+            ranked = []
+            for v in computed:
+                if v in M or nodes[v].op != "arith":
+                    continue
+                E = sum(len(c) for c in lc.get(v, {}).values())
+                gain = (E - 1) * lcost[v]
+                if E >= 2 and gain >= MAT_GAIN:
+                    ranked.append((gain, v))
+            if not ranked:
+                break
+            ranked.sort(reverse=True)
+            added = False
+            if limit is None:
+                M = set(M) | {ranked[0][1]}
                 continue
-            E = sum(len(c) for c in lc.get(v, {}).values())
-            gain = (E - 1) * lcost[v]
-            if E >= 2 and gain >= MAT_GAIN:
-                ranked.append((gain, v))
-        if not ranked:
-            break
-        ranked.sort(reverse=True)
-        added = False
-        for gain, v in ranked[:4]:
-            trial = set(M) | {v}
-            if layout(trial)[6] <= budget or layout(M)[6] > budget:
-                if layout(trial)[6] <= max(budget, layout(M)[6]):
-                    M = trial
-                    added = True
-                    break
-        if not added:
-            break
+            for gain, v in ranked[:4]:
+                trial = set(M) | {v}
+                if layout(trial)[6] <= limit or layout(M)[6] > limit:
+                    if layout(trial)[6] <= max(limit, layout(M)[6]):
+                        M = trial
+                        added = True
+                        break
+            if not added:
+                break
+        return M
+
+    def work(M):
+        """Per-point arithmetic a launch with materialised set M evaluates, summed over its
+        phases' boxes (each evaluation of a value at a cursor counts once)."""
+        _, out_phase, _, lc, phases, _, _ = layout(M)
+        vol = {}
+        for ph in phases:
+            v_ = 1
+            for ax, (lo, hi) in enumerate(ph.box):
+                v_ *= tile[ax] + hi - lo
+            vol[OUT if ph.nodes == [OUT] else ph.index] = v_
+        total = 0
+        for v, ctxs in lc.items():
+            if nodes[v].op != "arith":
+                continue
+            for ctx, curs in ctxs.items():
+                total += len(curs) * vol.get(ctx, 0)
+        return total
+
+    M_fit = greedy(set(M), budget, 64)
+    if budget is not None:
+        # when the LDS budget stopped the greedy, compare with materialising without limit:
+        # the launch is then cut into several (values crossing a cut go through HBM),
+        # which pays off when the expansion left in registers is much larger
+        M_free = greedy(set(M_fit), None, 4096)
+        if len(M_free) > len(M_fit):
+            w_fit, w_free = work(M_fit), work(M_free)
+            if SCHED_DEBUG:
+                print("schedule L%d: fit %d values work %d, free %d values work %d"
+                      % (L, len(M_fit), w_fit, len(M_free), w_free))
+            if w_fit > 2 * w_free:
+                M_fit = M_free
+    M = M_fit
     # values that ended up used once gain nothing from LDS
     phase, out_phase, members, lc = analyse(M)
     for v in sorted(M):

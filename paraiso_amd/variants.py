@@ -33,9 +33,13 @@ def _hydro_legacy():
 
 
 def _hydro(real, gamma=Fraction(7, 5), dt_mode="cfl"):
+    """Hydro with, beside ``proceed``, ``proceed_prime`` + ``proceed_pipelined``: the CFL
+    minimum of the next step computed in the store phase of this one (one grid pass)."""
     def mk():
         from .schemes.hydro import hydro_om
-        return hydro_om(real, gamma, dt_mode)
+        from .transform import pipeline_reduce
+        om = hydro_om(real, gamma, dt_mode)
+        return pipeline_reduce(om, "proceed") if dt_mode == "cfl" else om
     return mk
 
 

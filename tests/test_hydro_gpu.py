@@ -87,3 +87,40 @@ def test_uniform_state_is_fixed_point(require_gpu):
     m.run("proceed", 10)
     assert np.allclose(m.get("density"), 1.3, rtol=0, atol=1e-14)
     assert np.allclose(m.get("pressure"), 0.7, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("variant,real", [("hydro_f64", CType.Double),
+                                          ("hydro_f32_precise", CType.Float)])
+def test_pipelined_step_bit_exact(require_gpu, variant, real):
+    """proceed_prime + n x proceed_pipelined (the CFL minimum of the next step folded into
+    this step's store phase) stores what n x proceed stores, bit for bit."""
+    nx, ny, steps = 80, 56, 15
+    print("pipelined %s %dx%d steps=%d" % (variant, nx, ny, steps), flush=True)
+    dt = np.float32 if real is CType.Float else np.float64
+    a = variants.machine(variant, (nx, ny))
+    b = variants.machine(variant, (nx, ny))
+    ora = oracle_machine(lambda: hydro_om(real, GAMMA), (nx, ny))
+    for m in (a, b, ora):
+        setup_machine(m, nx, ny, 13, dt)
+    a.run("proceed", steps)
+    ora.run("proceed", steps)
+    b.run("proceed_prime")
+    b.run("proceed_pipelined", steps)
+    assert a.get("time") == b.get("time") == ora.get("time")
+    for f in FIELDS:
+        assert np.array_equal(a.get(f), b.get(f)), f
+        assert np.array_equal(b.get(f), ora.get(f)), f
+
+
+def test_pipelined_fast_f32_matches_proceed(require_gpu):
+    nx, ny, steps = 96, 64, 30
+    a = variants.machine("hydro_f32", (nx, ny))
+    b = variants.machine("hydro_f32", (nx, ny))
+    for m in (a, b):
+        setup_machine(m, nx, ny, 5, np.float32)
+    a.run("proceed", steps)
+    b.run("proceed_prime")
+    b.run("proceed_pipelined", steps, graph=True)
+    assert abs(a.get("time") - b.get("time")) <= 1e-6 * a.get("time")
+    for f in FIELDS:
+        assert rel_l1(a.get(f), b.get(f)) < 1e-6, f
