@@ -40,7 +40,9 @@ def conserved(m, gamma=1.4):
 
 def time_machine(variant, nx, ny, steps, warmup, seed, dtype, bytes_per_cell,
                  kernel="proceed_pipelined", prime="proceed_prime"):
-    """Warm up, then time `steps` calls of `kernel` on one stream (event pair per launch).
+    """Warm up, then time `steps` calls of `kernel` replayed as one HIP graph (wall clock
+    between device synchronisations); another `steps` calls with an event pair around every
+    launch give the device time of each launch.
 
     The default step is ``proceed_pipelined``: ``proceed`` with the CFL minimum for the
     next step reduced in this step's store phase (``prime`` computes the first one
@@ -52,14 +54,16 @@ def time_machine(variant, nx, ny, steps, warmup, seed, dtype, bytes_per_cell,
     if prime:
         m.run(prime)
     m.run(kernel, warmup)
+    m.run(kernel, steps, graph=True)             # builds and warms the graph
     m.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    launch_ms = m.run_timed(kernel, steps)
+    m.run(kernel, steps, graph=True)             # the timed steps: one HIP graph
     m.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    launch_ms = m.run_timed(kernel, steps)       # device time per launch (events)
     cells = nx * ny
     dom = max(range(len(launch_ms)), key=lambda l: launch_ms[l])
     avg_ms = launch_ms[dom] / steps

@@ -16,7 +16,8 @@
  * driven from different host threads and streams.  All work of a handle is
  * issued on its stream (pr_set_stream; a fresh non-blocking stream of default
  * priority otherwise).  Array statics are dense, x-fastest (index
- * x + nx*(y + ny*z)), with no margins on Cyclic axes.
+ * x + nx*(y + ny*z)) over the memory extent: the local size on Cyclic axes,
+ * the local size plus the plan's lower and upper margins on Open axes.
  */
 #ifndef PARAISO_HIP_H
 #define PARAISO_HIP_H
@@ -28,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PR_ABI_VERSION 3
+#define PR_ABI_VERSION 4
 
 enum pr_status {
   PR_OK = 0,
@@ -97,6 +98,10 @@ typedef struct pr_kernel_desc {
   const pr_launch_desc* launches;
   int n_stored;            /* array statics this kernel stores (ping-pong flipped after it) */
   const int* stored;
+  int n_loaded;            /* statics this kernel reads */
+  const int* loaded;
+  int n_written;           /* statics this kernel stores, arrays and scalars */
+  const int* written;
 } pr_kernel_desc;
 
 typedef struct pr_machine_desc {
@@ -170,7 +175,9 @@ long long pr_batch_cells(const pr_batch* b);
  * [S, S+every), and run kernel `adj` once per state in reverse order.  With
  * the adjoint statics holding dJ/du_N on entry they hold dJ/du_0 on return,
  * and the snapshotted statics hold u_0.  Needs ceil(nsteps/every) + every
- * slots. */
+ * slots.  When `adj` reads none of the snapshotted statics that `fwd` writes
+ * (a linear scheme: its adjoint does not depend on the state), the replay is
+ * skipped and `adj` runs `nsteps` times back to back. */
 int pr_snapshot_alloc(pr_machine* m, int n_slots, const int* statics, int n_statics);
 int pr_snapshot_save(pr_machine* m, int slot);
 int pr_snapshot_load(pr_machine* m, int slot);

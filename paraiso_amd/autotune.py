@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from typing import Dict, List, Tuple
@@ -59,10 +60,14 @@ def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[in
     def one(c):
         tile, nt = tuple(c[0]), int(c[1])
         name = candidate_name(variant, tile, nt)
-        return tile, nt, build_machine(setup.with_(tile=tile, threads=nt), mk(), name,
-                                       precise=precise)
+        try:
+            return tile, nt, build_machine(setup.with_(tile=tile, threads=nt), mk(), name,
+                                           precise=precise)
+        except (ValueError, subprocess.CalledProcessError) as e:
+            print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
+            return tile, nt, None
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        return list(ex.map(one, cands))
+        return [r for r in ex.map(one, cands) if r[2] is not None]
 
 
 def _setup_inputs(variant: str, m, size):
@@ -149,6 +154,7 @@ def main(argv=None):
     q.add_argument("--variant", action="append", required=True)
     q.add_argument("--size", type=int, nargs="+", action="append", required=True)
     q.add_argument("--steps", type=int, default=50)
+    q.add_argument("--kernel", default=None, help="kernel to time (default: the step)")
     q.add_argument("--out", required=True)
     s = sub.add_parser("select")
     s.add_argument("path")
@@ -160,7 +166,7 @@ def main(argv=None):
         out = {}
         for variant, size in zip(a.variant, a.size):
             out[variant] = {"size": size, "steps": a.steps,
-                            "results": measure(variant, size, a.steps)}
+                            "results": measure(variant, size, a.steps, kernel=a.kernel)}
             with open(a.out, "w") as f:
                 json.dump(out, f, indent=1)
     else:

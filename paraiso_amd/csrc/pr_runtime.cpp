@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <algorithm>
 #include <map>
 #include <tuple>
 #include <string>
@@ -444,6 +445,8 @@ int pr_snapshot_load(pr_machine* m, int slot) {
 
 int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {
   if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
+  if (fwd < 0 || fwd >= m->d->n_kernel)
+    return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
   const int S = (nsteps + every - 1) / every;
   if ((int)m->snaps.size() < S + every)
     return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
@@ -457,10 +460,26 @@ int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {
 
 int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every) {
   if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
+  if (fwd < 0 || fwd >= m->d->n_kernel || adj < 0 || adj >= m->d->n_kernel)
+    return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
   const int S = (nsteps + every - 1) / every;
   if ((int)m->snaps.size() < S + every)
     return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
   int rc;
+  // a linear scheme's adjoint reads nothing the forward kernel writes: no replay needed
+  const pr_kernel_desc& kf = m->d->kernels[fwd];
+  const pr_kernel_desc& ka = m->d->kernels[adj];
+  bool needs_state = false;
+  for (int i = 0; i < ka.n_loaded && !needs_state; ++i)
+    for (int j = 0; j < kf.n_written && !needs_state; ++j)
+      if (ka.loaded[i] == kf.written[j] &&
+          std::find(m->snap_sel.begin(), m->snap_sel.end(), ka.loaded[i]) != m->snap_sel.end())
+        needs_state = true;
+  if (!needs_state) {
+    for (int j = 0; j < nsteps; ++j)
+      if ((rc = launch_kernel_once(m, adj))) return rc;
+    return S > 0 ? pr_snapshot_load(m, 0) : PR_OK;
+  }
   for (int seg = S - 1; seg >= 0; --seg) {
     const int start = seg * every;
     const int len = (nsteps - start < every) ? nsteps - start : every;

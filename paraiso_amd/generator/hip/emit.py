@@ -592,9 +592,18 @@ class MachineEmitter:
             if stored:
                 launchers.append("static const int k_stored_%d[] = {%s};" % (
                     ki, ", ".join(str(s) for s in stored)))
-            kdescs.append('{"%s", %d, k_launch_%d, %d, %s}' % (
+            loaded = sorted(set(n.payload for n in sch.kernel.nodes
+                                if n.op == "load" and n.payload < self.n_user))
+            written = sorted(set(s for s, _ in sch.kernel.stores))
+            for nm, xs in (("loaded", loaded), ("written", written)):
+                if xs:
+                    launchers.append("static const int k_%s_%d[] = {%s};" % (
+                        nm, ki, ", ".join(str(x) for x in xs)))
+            kdescs.append('{"%s", %d, k_launch_%d, %d, %s, %d, %s, %d, %s}' % (
                 sch.kernel.name, len(sch.launches), ki, len(stored),
-                ("k_stored_%d" % ki) if stored else "nullptr"))
+                ("k_stored_%d" % ki) if stored else "nullptr",
+                len(loaded), ("k_loaded_%d" % ki) if loaded else "nullptr",
+                len(written), ("k_written_%d" % ki) if written else "nullptr"))
         out.extend(launchers)
         sdesc = ['{"%s", %s, %s, %d}' % (s.name, "PR_ARRAY" if s.dyn.realm is Realm.Array else "PR_SCALAR",
                                          PR_CTYPE[s.dyn.ctype], 1 if i >= self.n_user else 0)

@@ -180,7 +180,16 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, z
     la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, budget)
     la.kernel_override = k
     inner = [ph for ph in la.phases[:-1] if ph.index > 0]
+    if la.lds_bytes > budget and (len(inner) < 2 or depth > 6):
+        # cannot be cut further: keep to what the budget allows
+        la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero,
+                              budget, allow_free=False)
+        la.kernel_override = k
+        inner = [ph for ph in la.phases[:-1] if ph.index > 0]
     if la.lds_bytes <= budget or len(inner) < 2 or depth > 6:
+        if la.lds_bytes > LDS_MAX:
+            raise ValueError("a launch of kernel %s needs %d bytes of LDS at tile %s (at most %d)"
+                             % (k.name, la.lds_bytes, tuple(tile), LDS_MAX))
         return [la]
     nodes = k.nodes
     # cut where half of the computed materialised values are behind us
@@ -222,7 +231,7 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, z
 
 
 def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, costs,
-                     tile, zero, budget=LDS_BUDGET) -> ArrayLaunch:
+                     tile, zero, budget=LDS_BUDGET, allow_free=True) -> ArrayLaunch:
     nodes = k.nodes
     nd = k.ndim
     # cone of array values needed by this launch
@@ -444,7 +453,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         return total
 
     M_fit = greedy(set(M), budget, 64)
-    if budget is not None:
+    if budget is not None and allow_free:
         # when the LDS budget stopped the greedy, compare with materialising without limit:
         # the launch is then cut into several (values crossing a cut go through HBM),
         # which pays off when the expansion left in registers is much larger

@@ -36,7 +36,9 @@ class PrLaunchDesc(ctypes.Structure):
 class PrKernelDesc(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("n_launch", ctypes.c_int),
                 ("launches", ctypes.POINTER(PrLaunchDesc)), ("n_stored", ctypes.c_int),
-                ("stored", ctypes.POINTER(ctypes.c_int))]
+                ("stored", ctypes.POINTER(ctypes.c_int)), ("n_loaded", ctypes.c_int),
+                ("loaded", ctypes.POINTER(ctypes.c_int)), ("n_written", ctypes.c_int),
+                ("written", ctypes.POINTER(ctypes.c_int))]
 
 
 class PrMachineDesc(ctypes.Structure):

@@ -42,9 +42,31 @@ def test_machine_library_describes_its_program(name):
     lib = ctypes.CDLL(so)
     lib.pm_describe.restype = ctypes.POINTER(PrMachineDesc)
     d = lib.pm_describe().contents
-    assert d.abi == 3
+    assert d.abi == 4
     mk, setup, _ = variants.VARIANTS[name]
     om = mk()
     user = [d.statics[i].name.decode() for i in range(d.n_static) if not d.statics[i].flags & 1]
     assert user == [s.name for s in om.setup.static_values]
     assert [d.kernels[i].name.decode() for i in range(d.n_kernel)] == [k.name for k in om.kernels]
+
+
+def _kernel_statics(name, kernel, field):
+    lib = ctypes.CDLL(variants.library(name))
+    lib.pm_describe.restype = ctypes.POINTER(PrMachineDesc)
+    d = lib.pm_describe().contents
+    names = [d.statics[i].name.decode() for i in range(d.n_static)]
+    k = next(d.kernels[i] for i in range(d.n_kernel) if d.kernels[i].name.decode() == kernel)
+    n = getattr(k, "n_" + field)
+    return {names[getattr(k, field)[i]] for i in range(n)}
+
+
+def test_descriptors_list_what_kernels_read_and_write():
+    """The adjoint driver skips the forward replay when the adjoint kernel reads nothing the
+    forward kernel writes: true of the linear wave, not of Hydro."""
+    wave_adj = _kernel_statics("wave3d_f32", "proceed_adjoint", "loaded")
+    wave_fwd = _kernel_statics("wave3d_f32", "proceed", "written")
+    assert {"fieldF", "fieldG"} <= wave_fwd and "energy" in wave_fwd
+    assert not (wave_adj & wave_fwd) and {"adj_fieldF", "adj_fieldG"} <= wave_adj
+    hyd_adj = _kernel_statics("hydro_adj_f32", "proceed_adjoint", "loaded")
+    hyd_fwd = _kernel_statics("hydro_adj_f32", "proceed", "written")
+    assert {"density", "pressure"} <= (hyd_adj & hyd_fwd)
