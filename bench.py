@@ -78,6 +78,62 @@ def time_machine(variant, nx, ny, steps, warmup, seed, dtype, bytes_per_cell,
     }
 
 
+VALU_PEAK = 256 * 4 * 2.4e9 / 4      # wave64 VALU instructions/s, gfx950 without packed FP32
+
+
+def profile_figures(variant, kernel):
+    """HBM bytes and VALU instructions per launch of `kernel`, from the rocprofv3 summary
+    committed under profiles/ for this workload, if it was taken with the same tile."""
+    path = os.path.join(ROOT, "profiles", "hydro2d_kh_2048x2048_fp32.md")
+    if not os.path.exists(path):
+        return None, None
+    from paraiso_amd import variants
+    rec = {}
+    with open(path) as f:
+        for line in f:
+            for key in ("profile_config", "traffic_bytes_per_launch",
+                        "valu_wave_instructions_per_launch"):
+                if line.startswith(key + ":"):
+                    rec[key] = json.loads(line.split(":", 1)[1])
+    cfg = rec.get("profile_config", {})
+    setup = variants.setup_of(variant)
+    if (cfg.get("variant") != variant or list(setup.tile or ()) != cfg.get("tile")
+            or setup.threads != cfg.get("threads")):
+        return None, None
+    return (rec.get("traffic_bytes_per_launch", {}).get(kernel),
+            rec.get("valu_wave_instructions_per_launch", {}).get(kernel))
+
+
+def accuracy(nx, ny, steps=500, seed=1):
+    """FP32 (fast) against FP64 after `steps` steps from the same KH state: relative L1 of the
+    conserved state and per field (L1 and Linf), and drift of total mass and energy."""
+    import numpy as np
+    from paraiso_amd import variants
+    out, totals = {}, {}
+    ms = {}
+    for name, dt in (("hydro_f32", np.float32), ("hydro_f64", np.float64)):
+        m = variants.machine(name, (nx, ny))
+        setup_hydro(m, nx, ny, seed, dt)
+        c0 = conserved(m)
+        m.run("proceed_prime")
+        m.run("proceed_pipelined", steps)
+        ms[name] = m
+        c = conserved(m)
+        totals[name] = {"mass_drift": float(abs(c[0].sum() - c0[0].sum()) / c0[0].sum()),
+                        "energy_drift": float(abs(c[3].sum() - c0[3].sum()) / c0[3].sum())}
+    a, b = ms["hydro_f32"], ms["hydro_f64"]
+    ca, cb = conserved(a), conserved(b)
+    out["steps"] = steps
+    out["rel_l1_conserved"] = float(np.abs(ca - cb).sum() / np.abs(cb).sum())
+    for f in FIELDS:
+        x, y = a.get(f).astype(np.float64), b.get(f)
+        out[f] = {"rel_l1": float(np.abs(x - y).sum() / np.abs(y).sum()),
+                  "linf": float(np.abs(x - y).max())}
+    out["time_f32"], out["time_f64"] = float(a.get("time")), float(b.get("time"))
+    out["drift_f32"], out["drift_f64"] = totals["hydro_f32"], totals["hydro_f64"]
+    return out
+
+
 def hbm_copy_gbps():
     import torch
     n = 1 << 28                                # 1 GiB of float32 each way
@@ -95,7 +151,8 @@ def hbm_copy_gbps():
 
 
 def bench_batched(n_grids=256, steps=200, seed=2024):
-    """256 Hydro grids, sides log-uniform on [64, 1024], FP32, one HIP graph of `steps` steps."""
+    """256 Hydro grids, sides log-uniform on [64, 1024], FP32, one HIP graph of `steps` steps
+    of the carried-CFL step (each a single launch over the tiles of every grid)."""
     import numpy as np
     import torch
     from paraiso_amd import inputs, variants
@@ -103,18 +160,20 @@ def bench_batched(n_grids=256, steps=200, seed=2024):
     b = variants.batch("hydro_f32", sizes)
     for g, (nx, ny) in enumerate(sizes):
         setup_hydro(b.grids[g], nx, ny, seed + g, np.float32)
-    b.run("proceed", 2, graph=False)
-    b.run("proceed", steps, graph=True)          # builds and warms the graph
+    b.run("proceed_prime")
+    b.run("proceed_pipelined", 2, graph=False)
+    b.run("proceed_pipelined", steps, graph=True)   # builds and warms the graph
     b.sync()
     for g, (nx, ny) in enumerate(sizes):
         setup_hydro(b.grids[g], nx, ny, seed + g, np.float32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b.run("proceed", steps, graph=True)
+    b.run("proceed_prime")                          # the first step's CFL minimum
+    b.run("proceed_pipelined", steps, graph=True)
     b.sync()
     t1 = time.perf_counter()
     cells = b.cells()
-    ms = b.run_timed("proceed", 10)
+    ms = b.run_timed("proceed_pipelined", 10)
     return {"workload": "hydro2d-kh-batched-%d-grids-fp32" % n_grids, "grids": n_grids,
             "cells_per_step": cells, "steps": steps,
             "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
@@ -168,6 +227,33 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
             "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
             "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
             "grad_density_l1": float(np.abs(g).sum())}
+
+
+def bench_life(n=1024, gens=1000, seed=1):
+    """Conway Life 1024^2 int32 periodic, Bernoulli(0.5) cells, 1000 generations (one graph)."""
+    import torch
+    from paraiso_amd import inputs, variants
+    m = variants.machine("life_i32", (n, n))
+    m.run("init")
+    m.set("cell", inputs.life_init((n, n), seed))
+    m.run("proceed", gens, graph=True)            # builds and warms the graph
+    m.run("init")
+    m.set("cell", inputs.life_init((n, n), seed))
+    m.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.run("proceed", gens, graph=True)
+    m.sync()
+    t1 = time.perf_counter()
+    pop, gen = int(m.get("population")), int(m.get("generation"))
+    ms = m.run_timed("proceed", 100)
+    cells = n * n
+    return {"workload": "life-%dx%d-int32-%d-generations" % (n, n, gens), "generations": gens,
+            "value": cells * gens / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
+            "ms_per_step": (t1 - t0) / gens * 1e3,
+            "launch_ms_per_step": [x / 100 for x in ms], "algorithmic_bytes_per_cell": 8,
+            "achieved_GBps": cells * 8 * gens / (t1 - t0) / 1e9,
+            "population": pop, "generation": gen}
 
 
 def bench_wave(n=512, steps=300, every=20):
@@ -327,19 +413,29 @@ def main():
                      "avg_launch_ms": r32["dominant_avg_ms"]},
         "launch_ms_per_step": r32["launch_ms_per_step"],
         "step_kernel": "proceed_pipelined",
+        "valu": None,
         "unfused": {"kernel": "proceed", "value": r32u["value"],
                     "ms_per_step": r32u["ms_per_step"],
                     "launch_ms_per_step": r32u["launch_ms_per_step"]},
     }
+    if (nx, ny) == (2048, 2048):
+        kname = out["roofline"]["kernel"]
+        traffic, valu = profile_figures("hydro_f32", kname)
+        out["roofline"]["traffic"] = traffic
+        if valu is not None:
+            rate = valu / (r32["dominant_avg_ms"] * 1e-3)
+            out["valu"] = {"wave_instructions_per_launch": valu, "achieved_per_s": rate,
+                           "peak_per_s": VALU_PEAK, "frac": rate / VALU_PEAK,
+                           "source": "profiles/hydro2d_kh_2048x2048_fp32.md (SQ_INSTS_VALU)"}
     if not args.no_fp64:
         m64, r64 = time_machine("hydro_f64", nx, ny, K, W, 1, np.float64, 64)
-        c32, c64 = conserved(m32), conserved(m64)
-        err = float(np.abs(c32 - c64).sum() / np.abs(c64).sum())
         out["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                        "achieved_GBps": r64["achieved_GBps"],
                        "frac": r64["achieved_GBps"] / HBM_PEAK_GBPS,
                        "algorithmic_bytes_per_cell": 64}
-        out["fp32_vs_fp64_rel_l1_conserved"] = err
+        acc = accuracy(nx, ny)
+        out["fp32_vs_fp64"] = acc
+        out["fp32_vs_fp64_rel_l1_conserved"] = acc["rel_l1_conserved"]
         out["fp32_over_fp64_speedup"] = r32["value"] / r64["value"]
     out["hbm_copy_GBps"] = hbm_copy_gbps()
     if not args.no_extras:
@@ -347,6 +443,7 @@ def main():
         out["adjoint"] = bench_adjoint()
         out["fwd_adj_over_fwd"] = out["adjoint"]["fwd_adj_over_fwd"]
         out["wave3d"] = bench_wave()
+        out["life"] = bench_life()
     else:
         out["fwd_adj_over_fwd"] = None
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(nx, ny)

@@ -23,7 +23,7 @@ def _cases():
     from paraiso_amd.schemes.hydro import hydro_om
     cyc2 = (A.Condition.Cyclic,) * 2
     out = []
-    for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96)]:
+    for n in [(64, 64), (100, 37), (8, 8), (256, 128), (1024, 1024), (96, 96), (256, 192)]:
         out.append((life_om, n, cyc2))
     out.append((lambda: hydro_om(CType.Float, G), (2048, 2048), cyc2))   # bench CPU baseline
     open2 = (A.Condition.Open,) * 2

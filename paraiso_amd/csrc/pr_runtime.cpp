@@ -382,6 +382,79 @@ int pr_memory_size(const pr_machine* m, int axis) {
 
 long long pr_cells(const pr_machine* m) { return m ? m->cells : 0; }
 
+}  // extern "C"
+
+namespace {
+
+// One launch copies every static of a snapshot (arrays and 8-byte scalar entries): a
+// checkpointed adjoint saves or restores a snapshot per time step, and one
+// hipMemcpyAsync per static would cost a copy launch each.
+constexpr int PR_COPY_MAX = 32;
+struct CopyJob {
+  char* dst;
+  const char* src;
+  unsigned long long n;
+};
+struct CopyBatch {
+  CopyJob job[PR_COPY_MAX];
+};
+
+__global__ void __launch_bounds__(256) pr_multi_copy(CopyBatch b) {
+  const CopyJob j = b.job[blockIdx.y];
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  const unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool vec = (((uintptr_t)j.dst | (uintptr_t)j.src) & 15) == 0;
+  const unsigned long long n16 = vec ? j.n / 16 : 0;
+  for (unsigned long long i = t; i < n16; i += stride)
+    reinterpret_cast<uint4*>(j.dst)[i] = reinterpret_cast<const uint4*>(j.src)[i];
+  for (unsigned long long i = n16 * 16 + t; i < j.n; i += stride) j.dst[i] = j.src[i];
+}
+
+int snapshot_copy(pr_machine* m, int slot, bool save) {
+  CopyBatch b;
+  int count = 0;
+  unsigned long long most = 0;
+  auto flush = [&]() -> int {
+    if (!count) return PR_OK;
+    unsigned long long blocks = (most / 16 + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(pr_multi_copy, dim3((unsigned)blocks, (unsigned)count), dim3(256), 0,
+                       m->stream, b);
+    if (hipPeekAtLastError() != hipSuccess) return fail(m, PR_E_HIP, "snapshot copy launch");
+    count = 0;
+    most = 0;
+    return PR_OK;
+  };
+  for (int s : m->snap_sel) {
+    char* live;
+    char* keep;
+    unsigned long long n;
+    if (m->d->statics[s].realm == PR_ARRAY) {
+      live = (char*)m->buf[m->cur[s]][s];
+      keep = (char*)m->snaps[slot][s];
+      n = pr_static_bytes(m, s);
+    } else {
+      live = m->scal + 8 * s;
+      keep = m->snap_scal[slot] + 8 * s;
+      n = 8;
+    }
+    b.job[count].dst = save ? keep : live;
+    b.job[count].src = save ? live : keep;
+    b.job[count].n = n;
+    if (n > most) most = n;
+    if (++count == PR_COPY_MAX) {
+      int rc = flush();
+      if (rc) return rc;
+    }
+  }
+  return flush();
+}
+
+}  // namespace
+
+extern "C" {
+
 int pr_snapshot_alloc(pr_machine* m, int n_slots, const int* statics, int n_statics) {
   if (!m || n_slots < 0 || n_statics < 0 || (n_statics > 0 && !statics)) return PR_E_INVALID;
   const int ns = m->d->n_static;
@@ -419,28 +492,12 @@ int pr_snapshot_alloc(pr_machine* m, int n_slots, const int* statics, int n_stat
 
 int pr_snapshot_save(pr_machine* m, int slot) {
   if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
-  for (int s : m->snap_sel) {
-    if (m->d->statics[s].realm == PR_ARRAY)
-      PR_HIP(m, hipMemcpyAsync(m->snaps[slot][s], m->buf[m->cur[s]][s], pr_static_bytes(m, s),
-                               hipMemcpyDeviceToDevice, m->stream));
-    else
-      PR_HIP(m, hipMemcpyAsync(m->snap_scal[slot] + 8 * s, m->scal + 8 * s, 8,
-                               hipMemcpyDeviceToDevice, m->stream));
-  }
-  return PR_OK;
+  return snapshot_copy(m, slot, /*save=*/true);
 }
 
 int pr_snapshot_load(pr_machine* m, int slot) {
   if (!m || slot < 0 || slot >= (int)m->snaps.size()) return PR_E_INVALID;
-  for (int s : m->snap_sel) {
-    if (m->d->statics[s].realm == PR_ARRAY)
-      PR_HIP(m, hipMemcpyAsync(m->buf[m->cur[s]][s], m->snaps[slot][s], pr_static_bytes(m, s),
-                               hipMemcpyDeviceToDevice, m->stream));
-    else
-      PR_HIP(m, hipMemcpyAsync(m->scal + 8 * s, m->snap_scal[slot] + 8 * s, 8,
-                               hipMemcpyDeviceToDevice, m->stream));
-  }
-  return PR_OK;
+  return snapshot_copy(m, slot, /*save=*/false);
 }
 
 int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {

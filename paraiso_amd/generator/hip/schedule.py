@@ -114,7 +114,8 @@ def _neg(v):
     return tuple(-x for x in v)
 
 
-def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET) -> KernelSchedule:
+def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET,
+                    mat_gain: int = MAT_GAIN) -> KernelSchedule:
     nd = k.ndim
     zero = tuple(0 for _ in range(nd))
     nodes = k.nodes
@@ -141,7 +142,6 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET)
     # scalar values evaluated in tails: level >= 1 and used by something live
     slot_scalars = [n.id for n in nodes if n.realm is Realm.Scalar and level[n.id] >= 1]
 
-    costs = None
     launches = []
     scratch: List[Tuple[int, CType]] = []
     for L in range(n_launch):
@@ -150,7 +150,7 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET)
         red_L = [r for r in reduces if level[nodes[r].args[0]] == L]
         tail = [s for s in slot_scalars if level[s] == L + 1]
         launches.extend(_schedule_level(k, L, targets, stores_L, red_L, tail,
-                                        L == n_launch - 1, costs, tile, zero, scratch, budget))
+                                        L == n_launch - 1, mat_gain, tile, zero, scratch, budget))
     for i, la in enumerate(launches):
         la.index = i
     halo = 0
@@ -170,19 +170,19 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET)
 SCRATCH_BASE = 1 << 20        # scratch arrays are numbered from here until the emitter places them
 
 
-def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, scratch,
+def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, scratch,
                     budget, depth=0) -> List[ArrayLaunch]:
     """Schedule one level; cut it into consecutive launches while LDS is over budget.
 
     A cut after phase q stores every materialised value computed at or before
     q and read after it into a scratch array (its exact bits); the second
     launch reads those arrays as halo tiles, like loads of statics."""
-    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero, budget)
+    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, budget)
     la.kernel_override = k
     inner = [ph for ph in la.phases[:-1] if ph.index > 0]
     if la.lds_bytes > budget and (len(inner) < 2 or depth > 6):
         # cannot be cut further: keep to what the budget allows
-        la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, costs, tile, zero,
+        la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
                               budget, allow_free=False)
         la.kernel_override = k
         inner = [ph for ph in la.phases[:-1] if ph.index > 0]
@@ -218,19 +218,19 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, costs, tile, z
         sidx = SCRATCH_BASE + len(scratch)
         scratch.append((v, nodes[v].ctype))
         first_stores.append((sidx, v))
-    first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, costs, tile, zero,
+    first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, mat_gain, tile, zero,
                             scratch, budget, depth + 1)
     # second launch: the same targets, with the cut values read back as loads
     sidx_of = {v: s for s, v in first_stores}
     nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of else n
               for n in nodes]
     k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
-    second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, costs, tile, zero,
+    second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
                              scratch, budget, depth + 1)
     return first + second
 
 
-def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, costs,
+def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, mat_gain,
                      tile, zero, budget=LDS_BUDGET, allow_free=True) -> ArrayLaunch:
     nodes = k.nodes
     nd = k.ndim
@@ -414,7 +414,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                     continue
                 E = sum(len(c) for c in lc.get(v, {}).values())
                 gain = (E - 1) * lcost[v]
-                if E >= 2 and gain >= MAT_GAIN:
+                if E >= 2 and gain >= mat_gain:
                     ranked.append((gain, v))
             if not ranked:
                 break
@@ -423,7 +423,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if limit is None:
                 M = set(M) | {ranked[0][1]}
                 continue
-            for gain, v in ranked[:4]:
+            for gain, v in ranked[:4] + [x for x in ranked[4:] if x[0] >= 2 * mat_gain][:12]:
                 trial = set(M) | {v}
                 if layout(trial)[6] <= limit or layout(M)[6] > limit:
                     if layout(trial)[6] <= max(limit, layout(M)[6]):

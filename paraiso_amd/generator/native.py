@@ -32,6 +32,10 @@ class Setup:
     steps_per_launch: int = 1
     batched: bool = False        # also emit the many-grids form of every launch
     threads: int = 256           # work-items per workgroup (a multiple of 64, at most 1024)
+    # per-point operations a value needed at several offsets must save to be computed once
+    # into LDS instead of per offset in registers: the Manifest/Delayed choice of
+    # DecideAllocation.hs:41-43 (AllocationChoice), as a tuning knob
+    mat_gain: int = 48
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)

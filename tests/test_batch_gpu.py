@@ -35,3 +35,23 @@ def test_batch_equals_single_grids(require_gpu, variant, dt):
         assert b.grids[g].get("time") == m.get("time")
         for f in F:
             assert np.array_equal(b.grids[g].get(f), m.get(f)), (g, f)
+
+
+def test_batch_pipelined_equals_single_grids(require_gpu):
+    """The carried-CFL step batched over uneven grids, replayed as one graph."""
+    steps = 9
+    b = variants.batch("hydro_f32", SIZES)
+    for g, (nx, ny) in enumerate(SIZES):
+        init(b.grids[g], nx, ny, 200 + g, np.float32)
+    b.run("proceed_prime")
+    b.run("proceed_pipelined", steps, graph=True)
+    b.sync()
+    for g, (nx, ny) in enumerate(SIZES):
+        print("batch pipelined grid %d %dx%d" % (g, nx, ny), flush=True)
+        m = variants.machine("hydro_f32", (nx, ny))
+        init(m, nx, ny, 200 + g, np.float32)
+        m.run("proceed_prime")
+        m.run("proceed_pipelined", steps)
+        assert b.grids[g].get("time") == m.get("time")
+        for f in F:
+            assert np.array_equal(b.grids[g].get(f), m.get(f)), (g, f)
