@@ -374,7 +374,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the batched, adjoint and 3-D wave measurements")
+                    help="skip the batched, adjoint, 3-D wave and Life measurements")
+    ap.add_argument("--extras", default="batched,adjoint,wave3d,life",
+                    help="comma-separated subset of the extra measurements to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -438,14 +440,15 @@ def main():
         out["fp32_vs_fp64_rel_l1_conserved"] = acc["rel_l1_conserved"]
         out["fp32_over_fp64_speedup"] = r32["value"] / r64["value"]
     out["hbm_copy_GBps"] = hbm_copy_gbps()
-    if not args.no_extras:
-        out["batched"] = bench_batched()
-        out["adjoint"] = bench_adjoint()
-        out["fwd_adj_over_fwd"] = out["adjoint"]["fwd_adj_over_fwd"]
-        out["wave3d"] = bench_wave()
-        out["life"] = bench_life()
-    else:
-        out["fwd_adj_over_fwd"] = None
+    extras = [] if args.no_extras else [x for x in args.extras.split(",") if x]
+    runners = {"batched": bench_batched, "adjoint": bench_adjoint, "wave3d": bench_wave,
+               "life": bench_life}
+    for x in extras:
+        if x not in runners:
+            print("unknown extra measurement %r" % x, file=sys.stderr)
+            sys.exit(2)
+        out[x] = runners[x]()
+    out["fwd_adj_over_fwd"] = out["adjoint"]["fwd_adj_over_fwd"] if "adjoint" in out else None
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(nx, ny)
     print(json.dumps(out))
 

@@ -35,7 +35,8 @@ CANDIDATES = {
         ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024)],
     3: [((32, 4, 4), 256), ((64, 2, 2), 256), ((64, 4, 2), 256), ((64, 4, 4), 256),
         ((32, 8, 4), 256), ((64, 8, 2), 256), ((32, 8, 8), 512), ((64, 4, 4), 512),
-        ((64, 8, 4), 512)],
+        ((64, 8, 4), 512), ((64, 8, 8), 1024), ((32, 16, 8), 1024), ((32, 8, 16), 1024),
+        ((64, 16, 4), 1024)],
 }
 
 

@@ -32,6 +32,7 @@ from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
 SCHED_DEBUG = False
+CUT_GAIN = 2.0               # unlimited materialisation (with cuts) must save this factor of work
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
 LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches are cut
 LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
@@ -463,7 +464,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if SCHED_DEBUG:
                 print("schedule L%d: fit %d values work %d, free %d values work %d"
                       % (L, len(M_fit), w_fit, len(M_free), w_free))
-            if w_fit > 2 * w_free:
+            if w_fit > CUT_GAIN * w_free:
                 M_fit = M_free
     M = M_fit
     # values that ended up used once gain nothing from LDS
