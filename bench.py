@@ -409,8 +409,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": r32["achieved_GBps"], "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": r32["achieved_GBps"] / HBM_PEAK_GBPS,
                      "traffic": None,
-                     "kernel": "om_k%d_l%d" % (m32.kernel_index("proceed_pipelined"),
-                                               r32["dominant_launch"]),
+                     "kernel": "%s_proceed_pipelined_l%d" % (m32.name, r32["dominant_launch"]),
                      "algorithmic_bytes_per_cell": 32,
                      "avg_launch_ms": r32["dominant_avg_ms"]},
         "launch_ms_per_step": r32["launch_ms_per_step"],

@@ -151,6 +151,13 @@ int pr_size(const pr_machine* m, int axis);
 int pr_memory_size(const pr_machine* m, int axis);
 long long pr_cells(const pr_machine* m);
 
+/* ---- whole-machine state (the paper's "save the entire machine state to a file" and
+ * "load from such save-points", paper/om.tex:679-703; the reference generator has no
+ * counterpart).  Every user static is written with its name, type and bytes; loading
+ * checks the machine, grid size and statics before changing anything. */
+int pr_state_save(pr_machine* m, const char* path);
+int pr_state_load(pr_machine* m, const char* path);
+
 /* ---- batches: many independent grids of one machine, of uneven sizes,
  * advanced by the same launches (no counterpart in the reference).  Grid g
  * is a full machine (pr_batch_grid) for static access; all grids share the

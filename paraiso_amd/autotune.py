@@ -27,12 +27,16 @@ from typing import Dict, List, Tuple
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "tuned.json")
 
-# (tile, work-items per workgroup).  More work-items per workgroup raise the LDS
-# budget, so larger tiles keep their intermediates in LDS and recompute less halo.
+# (tile, work-items per workgroup[, materialisation threshold]).  More work-items per
+# workgroup raise the LDS budget, so larger tiles keep their intermediates in LDS and
+# recompute less halo; the threshold is the Manifest/Delayed choice of each value
+# (DecideAllocation.hs:41-43), which the reference's genetic tuner also searched.
 CANDIDATES = {
     2: [((64, 8), 256), ((32, 8), 256), ((32, 16), 256), ((64, 4), 256), ((128, 4), 256),
         ((64, 16), 256), ((32, 32), 256), ((32, 16), 512), ((64, 16), 512), ((32, 32), 512),
-        ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024)],
+        ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024),
+        ((64, 32), 1024, 24), ((64, 32), 1024, 96), ((32, 16), 512, 24), ((32, 16), 512, 96),
+        ((32, 8), 256, 16), ((32, 8), 256, 24)],
     3: [((32, 4, 4), 256), ((64, 2, 2), 256), ((64, 4, 2), 256), ((64, 4, 4), 256),
         ((32, 8, 4), 256), ((64, 8, 2), 256), ((32, 8, 8), 512), ((64, 4, 4), 512),
         ((64, 8, 4), 512), ((64, 8, 8), 1024), ((32, 16, 8), 1024), ((32, 8, 16), 1024),
@@ -47,8 +51,14 @@ def tuned() -> Dict:
     return {}
 
 
-def candidate_name(variant: str, tile, threads: int = 256) -> str:
-    return "%s__t%s_n%d" % (variant, "x".join(str(t) for t in tile), threads)
+def candidate_name(variant: str, tile, threads: int = 256, mat_gain: int = 48) -> str:
+    g = "" if mat_gain == 48 else "_g%d" % mat_gain
+    return "%s__t%s_n%d%s" % (variant, "x".join(str(t) for t in tile), threads, g)
+
+
+def _unpack(c):
+    """(tile, threads[, mat_gain]) -> (tile, threads, mat_gain)."""
+    return tuple(c[0]), int(c[1]), int(c[2]) if len(c) > 2 else 48
 
 
 def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[int, ...], int, str]]:
@@ -59,16 +69,16 @@ def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[in
     cands = candidates or CANDIDATES[setup.ndim]
 
     def one(c):
-        tile, nt = tuple(c[0]), int(c[1])
-        name = candidate_name(variant, tile, nt)
+        tile, nt, gain = _unpack(c)
+        name = candidate_name(variant, tile, nt, gain)
         try:
-            return tile, nt, build_machine(setup.with_(tile=tile, threads=nt), mk(), name,
-                                           precise=precise)
+            return tile, nt, gain, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain),
+                                                 mk(), name, precise=precise)
         except (ValueError, subprocess.CalledProcessError) as e:
             print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
-            return tile, nt, None
+            return tile, nt, gain, None
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        return [r for r in ex.map(one, cands) if r[2] is not None]
+        return [r for r in ex.map(one, cands) if r[3] is not None]
 
 
 def _setup_inputs(variant: str, m, size):
@@ -107,12 +117,13 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
     mk, setup, _ = variants.VARIANTS[variant]
     cands = candidates or CANDIDATES[setup.ndim]
     results = []
-    for tile, nt in cands:
-        name = candidate_name(variant, tile, nt)
+    for c in cands:
+        tile, nt, gain = _unpack(c)
+        name = candidate_name(variant, tile, nt, gain)
         so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
         if not os.path.exists(so):
             continue
-        print("tune %s tile=%s threads=%d" % (variant, tile, nt), flush=True)
+        print("tune %s tile=%s threads=%d mat_gain=%d" % (variant, tile, nt, gain), flush=True)
         m = Machine(so, size)
         _setup_inputs(variant, m, size)
         step = kernel
@@ -123,7 +134,7 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
         m.run(step, warmup)
         m.sync()
         ms = m.run_timed(step, steps)
-        results.append({"tile": list(tile), "threads": nt, "kernel": step,
+        results.append({"tile": list(tile), "threads": nt, "mat_gain": gain, "kernel": step,
                         "ms_per_step": sum(ms) / steps,
                         "launch_ms": [x / steps for x in ms]})
         m.close()
@@ -137,9 +148,11 @@ def select(path: str) -> Dict:
     for variant, rec in data.items():
         best = min(rec["results"], key=lambda r: r["ms_per_step"])
         t[variant] = {"tile": best["tile"], "threads": best["threads"],
+                      "mat_gain": best.get("mat_gain", 48),
                       "ms_per_step": best["ms_per_step"],
                       "size": rec["size"], "steps": rec["steps"],
-                      "candidates": {"%s_n%d" % ("x".join(map(str, r["tile"])), r["threads"]):
+                      "candidates": {"%s_n%d_g%d" % ("x".join(map(str, r["tile"])), r["threads"],
+                                                     r.get("mat_gain", 48)):
                                      round(r["ms_per_step"], 5) for r in rec["results"]}}
     with open(TUNED, "w") as f:
         json.dump(t, f, indent=1, sort_keys=True)
@@ -161,8 +174,8 @@ def main(argv=None):
     s.add_argument("path")
     a = ap.parse_args(argv)
     if a.cmd == "prepare":
-        for tile, nt, so in prepare(a.variant):
-            print(tile, nt, so)
+        for tile, nt, gain, so in prepare(a.variant):
+            print(tile, nt, gain, so)
     elif a.cmd == "measure":
         out = {}
         for variant, size in zip(a.variant, a.size):

@@ -65,11 +65,27 @@ __device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= OM_NWA
   return r;   // every thread gets the block result
 }
 
-// Reduce `count` partials in a fixed order: thread t folds t, t+NT, ... then the tree.
+// Reduce `count` partials in a fixed order.  Thread t keeps eight accumulators: lane
+// k folds partials t + (8j + k) NT, so eight independent loads are in flight per thread
+// (a one-workgroup reduction of 10^4-10^5 partials is bound by load latency); then the
+// eight are folded in order and the workgroup tree finishes.
 template <class T, class Op>
 __device__ __forceinline__ T om_partials_reduce(const T* __restrict__ part, int count, T init,
                                                  Op op, T* scratch) {
-  T acc = init;
-  for (int i = threadIdx.x; i < count; i += OM_NT) acc = op(acc, part[i]);
-  return om_block_reduce(acc, op, scratch);
+  T acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = init;
+  int i = threadIdx.x;
+  for (; i + 7 * OM_NT < count; i += 8 * OM_NT) {
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * OM_NT];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = op(acc[k], v[k]);
+  }
+  for (int k = 0; i < count; i += OM_NT, ++k) acc[k] = op(acc[k], part[i]);
+  T r = acc[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) r = op(r, acc[k]);
+  return om_block_reduce(r, op, scratch);
 }

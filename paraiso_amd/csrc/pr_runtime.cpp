@@ -382,6 +382,97 @@ int pr_memory_size(const pr_machine* m, int axis) {
 
 long long pr_cells(const pr_machine* m) { return m ? m->cells : 0; }
 
+// ---------------------------------------------------------------- whole-machine state files
+// Layout: "PRSTATE1", then int32 abi, ndim, local[3], n_static and the machine name
+// (uint32 length + bytes); per user static: name, int32 realm, int32 ctype, uint64 bytes,
+// data.  Scratch statics are not part of the state.
+
+static bool put(FILE* f, const void* p, size_t n) { return std::fwrite(p, 1, n, f) == n; }
+static bool get(FILE* f, void* p, size_t n) { return std::fread(p, 1, n, f) == n; }
+static bool put_str(FILE* f, const char* s) {
+  const uint32_t n = (uint32_t)std::strlen(s);
+  return put(f, &n, 4) && put(f, s, n);
+}
+static bool get_str(FILE* f, std::string& s) {
+  uint32_t n;
+  if (!get(f, &n, 4) || n > (1u << 20)) return false;
+  s.resize(n);
+  return n == 0 || get(f, &s[0], n);
+}
+
+int pr_state_save(pr_machine* m, const char* path) {
+  if (!m || !path) return PR_E_INVALID;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(m, PR_E_INVALID, std::string("state save: cannot open ") + path);
+  const pr_machine_desc* d = m->d;
+  int32_t head[7] = {PR_ABI_VERSION, d->ndim, m->local[0], m->local[1], m->local[2], 0, 0};
+  for (int s = 0; s < d->n_static; ++s)
+    if (!(d->statics[s].flags & PR_STATIC_SCRATCH)) ++head[5];
+  bool ok = put(f, "PRSTATE1", 8) && put(f, head, 6 * 4) && put_str(f, d->name);
+  std::vector<char> host;
+  for (int s = 0; ok && s < d->n_static; ++s) {
+    const pr_static_desc& sd = d->statics[s];
+    if (sd.flags & PR_STATIC_SCRATCH) continue;
+    const uint64_t n = pr_static_bytes(m, s);
+    host.resize(n);
+    if (pr_static_read(m, s, host.data(), n, 0) != PR_OK) {
+      std::fclose(f);
+      return PR_E_HIP;
+    }
+    const int32_t rc[2] = {sd.realm, sd.ctype};
+    ok = put_str(f, sd.name) && put(f, rc, 8) && put(f, &n, 8) && put(f, host.data(), n);
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? PR_OK : fail(m, PR_E_INVALID, std::string("state save: write failed: ") + path);
+}
+
+int pr_state_load(pr_machine* m, const char* path) {
+  if (!m || !path) return PR_E_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(m, PR_E_INVALID, std::string("state load: cannot open ") + path);
+  const pr_machine_desc* d = m->d;
+  auto bad = [&](const std::string& why) {
+    std::fclose(f);
+    return fail(m, PR_E_INVALID, "state load: " + why);
+  };
+  char magic[8];
+  int32_t head[6];
+  std::string name;
+  if (!get(f, magic, 8) || std::memcmp(magic, "PRSTATE1", 8) || !get(f, head, 24) ||
+      !get_str(f, name))
+    return bad("not a state file");
+  if (head[0] != PR_ABI_VERSION) return bad("ABI version differs");
+  if (name != d->name || head[1] != d->ndim) return bad("saved by machine " + name);
+  for (int a = 0; a < 3; ++a)
+    if (head[2 + a] != m->local[a]) return bad("grid size differs");
+  int n_user = 0;
+  for (int s = 0; s < d->n_static; ++s)
+    if (!(d->statics[s].flags & PR_STATIC_SCRATCH)) ++n_user;
+  if (head[5] != n_user) return bad("static count differs");
+  // read everything before writing anything: a mismatch leaves the machine untouched
+  std::vector<std::vector<char>> data(d->n_static);
+  for (int s = 0; s < d->n_static; ++s) {
+    const pr_static_desc& sd = d->statics[s];
+    if (sd.flags & PR_STATIC_SCRATCH) continue;
+    std::string sname;
+    int32_t rc[2];
+    uint64_t n;
+    if (!get_str(f, sname) || !get(f, rc, 8) || !get(f, &n, 8)) return bad("truncated");
+    if (sname != sd.name || rc[0] != sd.realm || rc[1] != sd.ctype ||
+        n != pr_static_bytes(m, s))
+      return bad("static " + sname + " differs from " + sd.name);
+    data[s].resize(n);
+    if (!get(f, data[s].data(), n)) return bad("truncated");
+  }
+  std::fclose(f);
+  for (int s = 0; s < d->n_static; ++s) {
+    if (data[s].empty()) continue;
+    int rc = pr_static_write(m, s, data[s].data(), data[s].size(), 0);
+    if (rc) return rc;
+  }
+  return PR_OK;
+}
+
 }  // extern "C"
 
 namespace {

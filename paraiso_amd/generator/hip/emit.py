@@ -26,6 +26,7 @@ Per-point expressions follow the reference's SASA naming (``a<node>_<cursor>``,
 """
 from __future__ import annotations
 
+import re
 from typing import Dict, List, Optional, Set, Tuple
 
 from ...om.arith import Op, ReduceOp
@@ -168,6 +169,11 @@ class MachineEmitter:
                 self.slot_index[(ki, v)] = len(self.slot_index)
 
     # ------------------------------------------------------------ helpers
+    def fn_name(self, ki: int, launch: int) -> str:
+        """Kernel symbol: machine, OM kernel and launch, so that profiles name them."""
+        clean = lambda x: re.sub(r"[^A-Za-z0-9_]", "_", x)
+        return "%s_%s_l%d" % (clean(self.name), clean(self.schedules[ki].kernel.name), launch)
+
     def sidx(self, ki, s) -> int:
         return self.scratch_map.get((ki, s), s)
 
@@ -197,7 +203,7 @@ class MachineEmitter:
         nodes = k.nodes
         nd = self.ndim
         T = self.tile
-        fn = "om_k%d_l%d" % (ki, la.index)
+        fn = self.fn_name(ki, la.index)
         self._ki = ki
         has_work = bool(la.targets)
         L = la.index
@@ -579,13 +585,13 @@ class MachineEmitter:
                 launchers.append(hsrc)
                 tile = list(self.tile) if la.targets else [0, 0, 0]
                 if self.batched:
-                    ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d, pack_om_k%d_l%d, sizeof(Args_om_k%d_l%d), "
-                                 "launch_batched_om_k%d_l%d}" % (
-                                     ki, la.index, tile[0], tile[1], tile[2], len(la.reduces),
-                                     ki, la.index, ki, la.index, ki, la.index))
+                    fn = self.fn_name(ki, la.index)
+                    ldesc.append("{launch_%s, {%d, %d, %d}, %d, pack_%s, sizeof(Args_%s), "
+                                 "launch_batched_%s}" % (fn, tile[0], tile[1], tile[2],
+                                                         len(la.reduces), fn, fn, fn))
                 else:
-                    ldesc.append("{launch_om_k%d_l%d, {%d, %d, %d}, %d, nullptr, 0, nullptr}" % (
-                        ki, la.index, tile[0], tile[1], tile[2], len(la.reduces)))
+                    ldesc.append("{launch_%s, {%d, %d, %d}, %d, nullptr, 0, nullptr}" % (
+                        self.fn_name(ki, la.index), tile[0], tile[1], tile[2], len(la.reduces)))
             stored = sorted(set(s for s, _ in sch.array_stores))
             out_l = "static const pr_launch_desc k_launch_%d[] = {%s};" % (ki, ", ".join(ldesc))
             launchers.append(out_l)

@@ -60,6 +60,8 @@ def emit_host_class(om, setup, margins=None) -> str:
         w("  int om_upper_margin_%d() const { return %d; }" % (ax, um[ax]))
     for ki, k in enumerate(om.kernels):
         w("  void %s() { check(pr_run(m_, %d, 1), m_, \"%s\"); }" % (k.name, ki, k.name))
+    w("  void save(const char* path) { check(pr_state_save(m_, path), m_, \"save\"); }")
+    w("  void load(const char* path) { check(pr_state_load(m_, path), m_, \"load\"); }")
     for si, s in enumerate(om.setup.static_values):
         t = s.dyn.ctype.c_name
         if s.dyn.realm is Realm.Scalar:

@@ -52,6 +52,7 @@ class PrMachineDesc(ctypes.Structure):
 RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_last_error",
                    "pr_run", "pr_run_graph", "pr_run_timed", "pr_static_devptr", "pr_static_write",
                    "pr_static_read", "pr_static_bytes", "pr_size", "pr_memory_size", "pr_cells",
+                   "pr_state_save", "pr_state_load",
                    "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load",
                    "pr_adjoint_forward", "pr_adjoint_backward",
                    "pr_batch_create", "pr_batch_destroy", "pr_batch_grid", "pr_batch_run",
@@ -86,6 +87,8 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_static_bytes.restype = sz
     lib.pr_size.argtypes = [vp, i]
     lib.pr_memory_size.argtypes = [vp, i]
+    lib.pr_state_save.argtypes = [vp, ctypes.c_char_p]
+    lib.pr_state_load.argtypes = [vp, ctypes.c_char_p]
     lib.pr_cells.argtypes = [vp]
     lib.pr_cells.restype = ctypes.c_longlong
     lib.pr_snapshot_alloc.argtypes = [vp, i, ctypes.POINTER(i), i]
@@ -244,6 +247,14 @@ class Machine:
 
     def snapshot_load(self, slot: int):
         self._check(self.rt.pr_snapshot_load(self.h, slot), "snapshot_load")
+
+    def save(self, path: str):
+        """Write every static to `path` (pr_state_save)."""
+        self._check(self.rt.pr_state_save(self.h, os.fsencode(path)), "state save")
+
+    def load(self, path: str):
+        """Restore the statics from a file written by `save` of the same machine and size."""
+        self._check(self.rt.pr_state_load(self.h, os.fsencode(path)), "state load")
 
     def adjoint_forward(self, fwd, nsteps: int, every: int):
         self._check(self.rt.pr_adjoint_forward(self.h, self.kernel_index(fwd), nsteps, every),

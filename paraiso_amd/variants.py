@@ -97,7 +97,8 @@ def setup_of(name: str):
     mk, setup, precise = VARIANTS[name]
     rec = tuned().get(name)
     if rec is not None:
-        setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)))
+        setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)),
+                            mat_gain=int(rec.get("mat_gain", 48)))
     return setup
 
 
