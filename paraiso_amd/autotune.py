@@ -36,11 +36,19 @@ CANDIDATES = {
         ((64, 16), 256), ((32, 32), 256), ((32, 16), 512), ((64, 16), 512), ((32, 32), 512),
         ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024),
         ((64, 32), 1024, 24), ((64, 32), 1024, 96), ((32, 16), 512, 24), ((32, 16), 512, 96),
-        ((32, 8), 256, 16), ((32, 8), 256, 24)],
+        ((32, 8), 256, 16), ((32, 8), 256, 24),
+        # streaming along y: the tile's y extent is the streaming length
+        ((64, 64), 256, 48, 4), ((128, 64), 512, 48, 4), ((128, 128), 1024, 48, 8),
+        ((256, 64), 1024, 48, 4), ((128, 64), 1024, 48, 8), ((64, 128), 512, 48, 8)],
     3: [((32, 4, 4), 256), ((64, 2, 2), 256), ((64, 4, 2), 256), ((64, 4, 4), 256),
         ((32, 8, 4), 256), ((64, 8, 2), 256), ((32, 8, 8), 512), ((64, 4, 4), 512),
         ((64, 8, 4), 512), ((64, 8, 8), 1024), ((32, 16, 8), 1024), ((32, 8, 16), 1024),
-        ((64, 16, 4), 1024)],
+        ((64, 16, 4), 1024),
+        # streaming along z
+        ((32, 8, 64), 256, 48, 1), ((64, 8, 64), 512, 48, 1), ((64, 16, 64), 1024, 48, 1),
+        ((32, 16, 64), 512, 48, 1), ((64, 8, 128), 512, 48, 2), ((32, 8, 128), 256, 48, 1),
+        ((64, 8, 128), 512, 48, 4), ((64, 16, 128), 1024, 48, 2), ((128, 8, 128), 1024, 48, 2),
+        ((64, 8, 256), 512, 48, 2), ((64, 4, 128), 256, 48, 2), ((128, 4, 128), 512, 48, 2)],
 }
 
 
@@ -51,14 +59,17 @@ def tuned() -> Dict:
     return {}
 
 
-def candidate_name(variant: str, tile, threads: int = 256, mat_gain: int = 48) -> str:
+def candidate_name(variant: str, tile, threads: int = 256, mat_gain: int = 48,
+                   march: int = 0) -> str:
     g = "" if mat_gain == 48 else "_g%d" % mat_gain
-    return "%s__t%s_n%d%s" % (variant, "x".join(str(t) for t in tile), threads, g)
+    mr = "_m%d" % march if march else ""
+    return "%s__t%s_n%d%s%s" % (variant, "x".join(str(t) for t in tile), threads, g, mr)
 
 
 def _unpack(c):
-    """(tile, threads[, mat_gain]) -> (tile, threads, mat_gain)."""
-    return tuple(c[0]), int(c[1]), int(c[2]) if len(c) > 2 else 48
+    """(tile, threads[, mat_gain[, march]]) -> (tile, threads, mat_gain, march)."""
+    return (tuple(c[0]), int(c[1]), int(c[2]) if len(c) > 2 else 48,
+            int(c[3]) if len(c) > 3 else 0)
 
 
 def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[int, ...], int, str]]:
@@ -69,16 +80,16 @@ def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[in
     cands = candidates or CANDIDATES[setup.ndim]
 
     def one(c):
-        tile, nt, gain = _unpack(c)
-        name = candidate_name(variant, tile, nt, gain)
+        tile, nt, gain, march = _unpack(c)
+        name = candidate_name(variant, tile, nt, gain, march)
         try:
-            return tile, nt, gain, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain),
-                                                 mk(), name, precise=precise)
+            return (c, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain, march=march),
+                                     mk(), name, precise=precise))
         except (ValueError, subprocess.CalledProcessError) as e:
             print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
-            return tile, nt, gain, None
+            return c, None
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        return [r for r in ex.map(one, cands) if r[3] is not None]
+        return [r for r in ex.map(one, cands) if r[1] is not None]
 
 
 def _setup_inputs(variant: str, m, size):
@@ -118,12 +129,13 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
     cands = candidates or CANDIDATES[setup.ndim]
     results = []
     for c in cands:
-        tile, nt, gain = _unpack(c)
-        name = candidate_name(variant, tile, nt, gain)
+        tile, nt, gain, march = _unpack(c)
+        name = candidate_name(variant, tile, nt, gain, march)
         so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
         if not os.path.exists(so):
             continue
-        print("tune %s tile=%s threads=%d mat_gain=%d" % (variant, tile, nt, gain), flush=True)
+        print("tune %s tile=%s threads=%d mat_gain=%d march=%d" % (variant, tile, nt, gain, march),
+              flush=True)
         m = Machine(so, size)
         _setup_inputs(variant, m, size)
         step = kernel
@@ -134,7 +146,8 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
         m.run(step, warmup)
         m.sync()
         ms = m.run_timed(step, steps)
-        results.append({"tile": list(tile), "threads": nt, "mat_gain": gain, "kernel": step,
+        results.append({"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
+                        "kernel": step,
                         "ms_per_step": sum(ms) / steps,
                         "launch_ms": [x / steps for x in ms]})
         m.close()
@@ -148,11 +161,12 @@ def select(path: str) -> Dict:
     for variant, rec in data.items():
         best = min(rec["results"], key=lambda r: r["ms_per_step"])
         t[variant] = {"tile": best["tile"], "threads": best["threads"],
-                      "mat_gain": best.get("mat_gain", 48),
+                      "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
                       "ms_per_step": best["ms_per_step"],
                       "size": rec["size"], "steps": rec["steps"],
-                      "candidates": {"%s_n%d_g%d" % ("x".join(map(str, r["tile"])), r["threads"],
-                                                     r.get("mat_gain", 48)):
+                      "candidates": {"%s_n%d_g%d_m%d" % ("x".join(map(str, r["tile"])),
+                                                         r["threads"], r.get("mat_gain", 48),
+                                                         r.get("march", 0)):
                                      round(r["ms_per_step"], 5) for r in rec["results"]}}
     with open(TUNED, "w") as f:
         json.dump(t, f, indent=1, sort_keys=True)
@@ -174,8 +188,8 @@ def main(argv=None):
     s.add_argument("path")
     a = ap.parse_args(argv)
     if a.cmd == "prepare":
-        for tile, nt, gain, so in prepare(a.variant):
-            print(tile, nt, gain, so)
+        for c, so in prepare(a.variant):
+            print(c, so)
     elif a.cmd == "measure":
         out = {}
         for variant, size in zip(a.variant, a.size):

@@ -45,12 +45,12 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
     budget = lds_budget(setup.threads)
     lowered = [lower_kernel(om_opt, ki, cyclic) for ki in range(len(om_opt.kernels))]
     regions = _target_regions(plan, lowered) if not all(cyclic) else {}
-    scheds = [schedule_kernel(k, tile, budget, setup.mat_gain) for k in lowered]
+    scheds = [schedule_kernel(k, tile, budget, setup.mat_gain, setup.march) for k in lowered]
     src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,
                          fast_fp32=not precise, batched=setup.batched,
                          threads=setup.threads, open_axes=tuple(not c for c in cyclic),
                          margins=(plan.lower_margin, plan.upper_margin),
-                         regions=regions).emit()
+                         regions=regions, march=setup.march).emit()
     hpp = emit_host_class(om_opt, setup, (plan.lower_margin, plan.upper_margin))
     return plan, src, hpp
 

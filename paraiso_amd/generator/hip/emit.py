@@ -33,7 +33,7 @@ from ...om.arith import Op, ReduceOp
 from ...om.dyn import CType, Realm
 from ...om.literal import c_literal
 from .ir import LKernel
-from .schedule import OUT, ArrayLaunch, KernelSchedule
+from .schedule import OUT, ArrayLaunch, KernelSchedule, ring_depth
 
 NT = 256
 PR_CTYPE = {CType.Bool: "PR_BOOL", CType.Int: "PR_INT", CType.Float: "PR_FLOAT",
@@ -131,9 +131,13 @@ class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
                  tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
                  threads: int = NT, open_axes: Tuple[bool, ...] = (), margins=None,
-                 regions=None):
+                 regions=None, march: int = 0):
         self.name = name
         self.nt = threads
+        # march > 0: a workgroup streams along the last axis through its tile, `march`
+        # planes per step, each materialised value kept as a ring of planes in LDS
+        self.march = march
+        self._m = None
         # Open (non-periodic) axes: grids carry lower/upper margins of ghost cells; a store or
         # reduce argument computed over region [lb, n - ub) holds 0 elsewhere
         self.open_axes = tuple(open_axes) + (False,) * (3 - len(open_axes))
@@ -254,7 +258,9 @@ class MachineEmitter:
             a_ct = nodes[nodes[r].args[0]].ctype
             at = acc_type(nodes[r].payload, a_ct)
             w("  %s acc%d = %s;" % (at.c_name, r, acc_init(nodes[r].payload, at)))
-        if has_work:
+        if has_work and self.march:
+            self._emit_march(k, la, w)
+        elif has_work:
             for ph in la.phases:
                 self._emit_phase(k, la, ph, w)
         # ---- epilogue: per-workgroup partials; the finish kernel does the rest
@@ -373,12 +379,49 @@ class MachineEmitter:
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"
 
+    def _emit_march(self, k: LKernel, la: ArrayLaunch, w):
+        """Stream along the last axis: at step s every phase computes the R planes at its
+        leading edge, [s R + hi, s R + hi + R) of its box (planes below the box are
+        skipped), then the store phase does planes [s R, s R + R).  A phase's ring holds
+        hi - lo + R planes, so what later phases and the next steps read is still there."""
+        R = self.march
+        ma = self.ndim - 1
+        Z = self.tile[ma]
+        if Z % R:
+            raise ValueError("the tile's last extent %d is not a multiple of march %d" % (Z, R))
+        inner = la.phases[:-1]
+        s0 = min([(ph.box[ma][0] - ph.box[ma][1]) // R for ph in inner] + [0])
+        w("  // march along axis %d: %d planes per step over %d planes" % (ma, R, Z))
+        w("  for (int s = %d; s < %d; ++s) {" % (s0, Z // R))
+        for ph in inner:
+            self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
+            self._emit_phase(k, la, ph, w)
+        self._m = {"R": R, "ma": ma, "lo": 0, "hi": 0}
+        w("  if (s >= 0) {")
+        self._emit_phase(k, la, la.phases[-1], w)
+        w("  }")
+        w("  }")
+        self._m = None
+
+    def _ring(self, la, v):
+        """(depth, plane size, bias) of value v's ring of planes."""
+        ma = self.ndim - 1
+        bb = tuple(la.lds[v].box) + ((0, 0),) * (3 - len(la.lds[v].box))
+        depth = ring_depth(bb[ma][1] - bb[ma][0] + self.march)
+        plane = 1
+        for ax in range(3):
+            if ax != ma:
+                plane *= self.tile[ax] + bb[ax][1] - bb[ax][0]
+        return depth, plane, depth * 64
+
     def _emit_phase(self, k: LKernel, la: ArrayLaunch, ph, w):
         nodes = k.nodes
         nd = self.ndim
         T = self.tile
         box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
         W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
+        if self._m:
+            W[self._m["ma"]] = self._m["R"]          # one batch of planes per step
         vol = W[0] * W[1] * W[2]
         is_out = ph.nodes == [OUT]
         w("  // phase %d: %s over box %s" % (ph.index, "store" if is_out else
@@ -417,8 +460,17 @@ class MachineEmitter:
         w(ind + "    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
         w(ind + "    const int l2 = p / %d;" % (W[0] * W[1]))
         w(ind + "    (void)l0; (void)l1; (void)l2;")
+        m = self._m
+        if m:
+            ma = m["ma"]
+            w(ind + "    const int jz = s * %d + %d + l%d;" % (m["R"], m["hi"], ma))
+            if not is_out:
+                w(ind + "    if (jz < %d) continue;" % m["lo"])
         for ax in range(3):
-            w(ind + "    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+            if m and ax == m["ma"]:
+                w(ind + "    const int g%d = X%d + jz; (void)g%d;" % (ax, ax, ax))
+            else:
+                w(ind + "    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
         if is_out:
             conds = ["g%d < a.n%d" % (ax, ax) for ax in range(nd)]
             w(ind + "    if (!(%s)) continue;" % " && ".join(conds))
@@ -443,8 +495,22 @@ class MachineEmitter:
                 w(ind + "    acc%d = %s(acc%d, %s);" % (r, acc_op(nodes[r].payload, at), r, val))
         else:
             for v in ph.nodes:
-                w(ind + "    s%d[p] = n%d_%s;" % (v, v, _cname(zero)))
+                w(ind + "    s%d[%s] = n%d_%s;" % (v, self._lds_self(la, v, W), v, _cname(zero)))
         w(ind + "  }")
+
+    def _lds_self(self, la, v, W) -> str:
+        """Where a phase stores its own value at the loop point."""
+        if not self._m:
+            return "p"
+        ma = self._m["ma"]
+        depth, plane, bias = self._ring(la, v)
+        terms, stride = [], 1
+        for ax in range(3):
+            if ax == ma:
+                continue
+            terms.append("l%d * %d" % (ax, stride))
+            stride *= W[ax]
+        return "((jz + %d) & %d) * %d + %s" % (bias, depth - 1, plane, " + ".join(terms))
 
     def _val_name(self, k, la, v, c, roots):
         return "n%d_%s" % (v, _cname(c))
@@ -495,8 +561,19 @@ class MachineEmitter:
                     UW = [self.tile[ax] + b[ax][1] - b[ax][0] for ax in range(3)]
                     cc = tuple(c) + (0,) * (3 - nd)
                     terms = ["l%d + %d" % (ax, box[ax][0] + cc[ax] - b[ax][0]) for ax in range(3)]
-                    idx = "(%s) * %d + (%s) * %d + (%s)" % (terms[2], UW[0] * UW[1], terms[1], UW[0],
-                                                            terms[0])
+                    if self._m:
+                        ma = self._m["ma"]
+                        depth, plane, bias = self._ring(la, v)
+                        parts, stride = [], 1
+                        for ax in range(3):
+                            if ax != ma:
+                                parts.append("(%s) * %d" % (terms[ax], stride))
+                                stride *= UW[ax]
+                        idx = "((jz + %d) & %d) * %d + %s" % (cc[ma] + bias, depth - 1, plane,
+                                                              " + ".join(parts))
+                    else:
+                        idx = "(%s) * %d + (%s) * %d + (%s)" % (terms[2], UW[0] * UW[1], terms[1],
+                                                                UW[0], terms[0])
                     lines.append("const %s %s = s%d[%s];" % (tn, name, v, idx))
                 elif n.op == "load":
                     lines.append("const %s %s = a.ld%d[%s];" % (tn, name, self.sidx(self._ki, n.payload),

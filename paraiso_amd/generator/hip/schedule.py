@@ -38,6 +38,14 @@ LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches 
 LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
 
 
+def ring_depth(planes: int) -> int:
+    """Planes of a streaming ring, rounded up to a power of two (slots are masked)."""
+    d = 1
+    while d < planes:
+        d *= 2
+    return d
+
+
 def lds_budget(threads: int) -> int:
     """LDS bytes a workgroup of ``threads`` may use: two resident workgroups per CU up to
     512 work-items (one can compute while the other waits at a barrier), one above."""
@@ -116,7 +124,9 @@ def _neg(v):
 
 
 def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET,
-                    mat_gain: int = MAT_GAIN) -> KernelSchedule:
+                    mat_gain: int = MAT_GAIN, march: int = 0) -> KernelSchedule:
+    """``march`` > 0: workgroups stream along the last axis, ``march`` planes per step, and
+    keep each materialised value as a ring of planes (see the emitter)."""
     nd = k.ndim
     zero = tuple(0 for _ in range(nd))
     nodes = k.nodes
@@ -151,7 +161,8 @@ def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET,
         red_L = [r for r in reduces if level[nodes[r].args[0]] == L]
         tail = [s for s in slot_scalars if level[s] == L + 1]
         launches.extend(_schedule_level(k, L, targets, stores_L, red_L, tail,
-                                        L == n_launch - 1, mat_gain, tile, zero, scratch, budget))
+                                        L == n_launch - 1, mat_gain, tile, zero, scratch, budget,
+                                        march=march))
     for i, la in enumerate(launches):
         la.index = i
     halo = 0
@@ -172,19 +183,20 @@ SCRATCH_BASE = 1 << 20        # scratch arrays are numbered from here until the 
 
 
 def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, scratch,
-                    budget, depth=0) -> List[ArrayLaunch]:
+                    budget, depth=0, march=0) -> List[ArrayLaunch]:
     """Schedule one level; cut it into consecutive launches while LDS is over budget.
 
     A cut after phase q stores every materialised value computed at or before
     q and read after it into a scratch array (its exact bits); the second
     launch reads those arrays as halo tiles, like loads of statics."""
-    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, budget)
+    la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, budget,
+                          march=march)
     la.kernel_override = k
     inner = [ph for ph in la.phases[:-1] if ph.index > 0]
     if la.lds_bytes > budget and (len(inner) < 2 or depth > 6):
         # cannot be cut further: keep to what the budget allows
         la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
-                              budget, allow_free=False)
+                              budget, allow_free=False, march=march)
         la.kernel_override = k
         inner = [ph for ph in la.phases[:-1] if ph.index > 0]
     if la.lds_bytes <= budget or len(inner) < 2 or depth > 6:
@@ -220,19 +232,19 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
         scratch.append((v, nodes[v].ctype))
         first_stores.append((sidx, v))
     first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, mat_gain, tile, zero,
-                            scratch, budget, depth + 1)
+                            scratch, budget, depth + 1, march)
     # second launch: the same targets, with the cut values read back as loads
     sidx_of = {v: s for s, v in first_stores}
     nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of else n
               for n in nodes]
     k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
     second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
-                             scratch, budget, depth + 1)
+                             scratch, budget, depth + 1, march)
     return first + second
 
 
 def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, mat_gain,
-                     tile, zero, budget=LDS_BUDGET, allow_free=True) -> ArrayLaunch:
+                     tile, zero, budget=LDS_BUDGET, allow_free=True, march=0) -> ArrayLaunch:
     nodes = k.nodes
     nd = k.ndim
     # cone of array values needed by this launch
@@ -392,11 +404,14 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         live: List[LdsBuf] = []
         top = 0
         for ph in phases[:-1]:
-            live = [b for b in live if last_read[b.node] >= ph.index]
+            if not march:
+                live = [b for b in live if last_read[b.node] >= ph.index]
+            # (a marching launch runs every phase at every step: its rings stay live)
             for v in ph.nodes:
                 vol = 1
                 for ax, (lo, hi) in enumerate(ph.box):
-                    vol *= tile[ax] + hi - lo
+                    vol *= (ring_depth(hi - lo + march) if (march and ax == nd - 1)
+                            else tile[ax] + hi - lo)
                 nbytes = (vol * nodes[v].ctype.size + 15) // 16 * 16
                 off = _first_fit(live, nbytes)
                 b = LdsBuf(v, off, nbytes, ph.box)
@@ -443,7 +458,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         for ph in phases:
             v_ = 1
             for ax, (lo, hi) in enumerate(ph.box):
-                v_ *= tile[ax] + hi - lo
+                v_ *= tile[ax] if (march and ax == nd - 1) else tile[ax] + hi - lo
             vol[OUT if ph.nodes == [OUT] else ph.index] = v_
         total = 0
         for v, ctxs in lc.items():

@@ -36,6 +36,9 @@ class Setup:
     # into LDS instead of per offset in registers: the Manifest/Delayed choice of
     # DecideAllocation.hs:41-43 (AllocationChoice), as a tuning knob
     mat_gain: int = 48
+    # > 0: each workgroup streams along the last axis through its tile (whose last extent
+    # is then the streaming length), this many planes per step
+    march: int = 0
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)

@@ -83,6 +83,14 @@ VARIANTS = {
                                                       tile=(32, 8)), True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
     "wave3d_f64": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP, boundary=CYC3), True),
+    # streaming along the last axis (rings of planes in LDS), checked bit for bit
+    "life_i32_march": (_life, Setup((64, 64), Language.HIP, boundary=CYC2, tile=(64, 32),
+                                    march=4), True),
+    "hydro_f64_march": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                    tile=(64, 32), march=4), True),
+    "wave3d_f64_march": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
+                                                           boundary=CYC3, tile=(32, 8, 32),
+                                                           march=1), True),
     # Open (non-periodic) boundaries: ghost margins, as the reference's shipped examples
     "life_open_i32": (_life, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
     "life_legacy_open_i32": (_life_legacy, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
@@ -98,7 +106,7 @@ def setup_of(name: str):
     rec = tuned().get(name)
     if rec is not None:
         setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)),
-                            mat_gain=int(rec.get("mat_gain", 48)))
+                            mat_gain=int(rec.get("mat_gain", 48)), march=int(rec.get("march", 0)))
     return setup
 
 

@@ -40,10 +40,12 @@ def run_pair(variant, real, nx, ny, steps, seed=11):
     return gpu, ora
 
 
-@pytest.mark.parametrize("variant,real", [("hydro_f64", CType.Double),
-                                          ("hydro_f32_precise", CType.Float)])
-def test_precise_bit_exact(require_gpu, variant, real):
-    gpu, ora = run_pair(variant, real, 64, 48, 20)
+@pytest.mark.parametrize("variant,real,nx,ny", [("hydro_f64", CType.Double, 64, 48),
+                                                ("hydro_f32_precise", CType.Float, 64, 48),
+                                                ("hydro_f64_march", CType.Double, 64, 48),
+                                                ("hydro_f64_march", CType.Double, 96, 64)])
+def test_precise_bit_exact(require_gpu, variant, real, nx, ny):
+    gpu, ora = run_pair(variant, real, nx, ny, 20)
     assert gpu.get("time") == ora.get("time")
     for f in FIELDS:
         g, o = gpu.get(f), ora.get(f)

@@ -13,10 +13,10 @@ from paraiso_amd.schemes.life import life_om
 pytestmark = pytest.mark.gpu
 
 
-def run_both(nx, ny, gens, seed):
-    print("life case nx=%d ny=%d gens=%d seed=%d" % (nx, ny, gens, seed), flush=True)
+def run_both(nx, ny, gens, seed, variant="life_i32"):
+    print("life case %s nx=%d ny=%d gens=%d seed=%d" % (variant, nx, ny, gens, seed), flush=True)
     cells = inputs.life_init((ny, nx), seed)
-    gpu = variants.machine("life_i32", (nx, ny))
+    gpu = variants.machine(variant, (nx, ny))
     gpu.run("init")
     gpu.set("cell", cells)
     ora = oracle_machine(life_om, (nx, ny))
@@ -27,10 +27,11 @@ def run_both(nx, ny, gens, seed):
     return gpu, ora
 
 
+@pytest.mark.parametrize("variant", ["life_i32", "life_i32_march"])
 @pytest.mark.parametrize("nx,ny,gens,seed", [(64, 64, 50, 1), (100, 37, 30, 2),
                                               (8, 8, 20, 3), (256, 128, 200, 4)])
-def test_life_matches_oracle(require_gpu, nx, ny, gens, seed):
-    gpu, ora = run_both(nx, ny, gens, seed)
+def test_life_matches_oracle(require_gpu, nx, ny, gens, seed, variant):
+    gpu, ora = run_both(nx, ny, gens, seed, variant)
     assert np.array_equal(gpu.get("cell"), ora.get("cell"))
     assert gpu.get("population") == ora.get("population")
     assert gpu.get("generation") == ora.get("generation") == gens

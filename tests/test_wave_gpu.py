@@ -21,11 +21,14 @@ def params(m, c=1.0, dx=1.0, dt=0.5):
     m.set("dt", dt)
 
 
-def test_wave3d_f64_bit_exact_vs_oracle(require_gpu):
-    n = (16, 12, 10)
+@pytest.mark.parametrize("variant,n", [("wave3d_f64", (16, 12, 10)),
+                                       ("wave3d_f64_march", (16, 12, 10)),
+                                       ("wave3d_f64_march", (40, 9, 70))])
+def test_wave3d_f64_bit_exact_vs_oracle(require_gpu, variant, n):
+    print("wave case %s %s" % (variant, n), flush=True)
     rng = np.random.default_rng(3)
     f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
-    gpu = variants.machine("wave3d_f64", n)
+    gpu = variants.machine(variant, n)
     ora = oracle_machine(lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
                                              ["fieldF", "fieldG"]), n)
     for m in (gpu, ora):
@@ -56,14 +59,15 @@ def test_wave3d_exact_plane_wave(require_gpu):
     assert np.abs(m.get("fieldF") - np.cos(ph - w * dt * steps)).max() < 1e-11
 
 
-def test_wave3d_adjoint_dot_product(require_gpu):
+@pytest.mark.parametrize("variant", ["wave3d_f64", "wave3d_f64_march"])
+def test_wave3d_adjoint_dot_product(require_gpu, variant):
     n = (16, 12, 10)
     nsteps, every = 7, 3
     rng = np.random.default_rng(5)
     x = [torch.tensor(rng.standard_normal(n[::-1]), device="cuda", requires_grad=True)
          for _ in range(2)]
     y = [torch.tensor(rng.standard_normal(n[::-1]), device="cuda") for _ in range(2)]
-    m = variants.machine("wave3d_f64", n)
+    m = variants.machine(variant, n)
     params(m)
     out = rollout(m, ["fieldF", "fieldG"], x, nsteps, every)
     lhs = sum((o * yy).sum() for o, yy in zip(out, y))
