@@ -379,8 +379,9 @@ def main():
                     help="comma-separated subset of the extra measurements to run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (grids would shard "
+              "by slabs along their slowest axis with one halo exchange per launch)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -221,6 +221,7 @@ class _SubEmitter:
             return "om_reduce_%s(%s)" % (inst.op.value.lower(), "a%d" % prepre[0]), []
         if isinstance(inst, Broadcast):
             return "om_broadcast(%s)" % ("a%d" % prepre[0]), []
+        # the reference's fallback for any other instruction (PlanTrans.hs:570)
         return "42 /* TODO : %s */" % inst, []
 
     def body(self) -> List[str]:
