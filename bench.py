@@ -157,7 +157,7 @@ def bench_batched(n_grids=256, steps=200, seed=2024):
     import torch
     from paraiso_amd import inputs, variants
     sizes = [tuple(int(x) for x in s) for s in inputs.batched_sizes(n_grids, seed)]
-    b = variants.batch("hydro_f32", sizes)
+    b = variants.batch("hydro_f32_batch", sizes)
     for g, (nx, ny) in enumerate(sizes):
         setup_hydro(b.grids[g], nx, ny, seed + g, np.float32)
     b.run("proceed_prime")

@@ -36,7 +36,7 @@ CANDIDATES = {
         ((64, 16), 256), ((32, 32), 256), ((32, 16), 512), ((64, 16), 512), ((32, 32), 512),
         ((128, 8), 512), ((64, 32), 1024), ((32, 64), 1024),
         ((64, 32), 1024, 24), ((64, 32), 1024, 96), ((32, 16), 512, 24), ((32, 16), 512, 96),
-        ((32, 8), 256, 16), ((32, 8), 256, 24),
+        ((32, 8), 256, 16), ((32, 8), 256, 24), ((32, 16), 256, 24), ((64, 8), 256, 24),
         # streaming along y: the tile's y extent is the streaming length
         ((64, 64), 256, 48, 4), ((128, 64), 512, 48, 4), ((128, 128), 1024, 48, 8),
         ((256, 64), 1024, 48, 4), ((128, 64), 1024, 48, 8), ((64, 128), 512, 48, 8)],
@@ -119,25 +119,34 @@ def _setup_inputs(variant: str, m, size):
 
 
 def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=None,
-            kernel: str = None) -> List[Dict]:
+            kernel: str = None, batch: int = 0, libs_of: str = None) -> List[Dict]:
     """Device time per step of every compiled candidate; the step is ``proceed_pipelined``
-    (after ``proceed_prime``) where the program has it, else ``kernel`` or ``proceed``."""
-    from . import variants
+    (after ``proceed_prime``) where the program has it, else ``kernel`` or ``proceed``.
+    ``batch`` > 0 times a batch of that many grids of log-uniform sides on [64, 1024]
+    (seed 2024) instead of one grid of ``size``; ``libs_of`` names the variant whose
+    compiled candidates are timed (default: ``variant``)."""
+    from . import inputs, variants
     from .generator import GEN_DIR
-    from .machine import Machine
+    from .machine import Batch, Machine
     mk, setup, _ = variants.VARIANTS[variant]
     cands = candidates or CANDIDATES[setup.ndim]
     results = []
     for c in cands:
         tile, nt, gain, march = _unpack(c)
-        name = candidate_name(variant, tile, nt, gain, march)
+        name = candidate_name(libs_of or variant, tile, nt, gain, march)
         so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
         if not os.path.exists(so):
             continue
         print("tune %s tile=%s threads=%d mat_gain=%d march=%d" % (variant, tile, nt, gain, march),
               flush=True)
-        m = Machine(so, size)
-        _setup_inputs(variant, m, size)
+        if batch:
+            sizes = [tuple(int(x) for x in sz) for sz in inputs.batched_sizes(batch, 2024)]
+            m = Batch(so, sizes)
+            for g, sz in enumerate(sizes):
+                _setup_inputs(variant, m.grids[g], sz)
+        else:
+            m = Machine(so, size)
+            _setup_inputs(variant, m, size)
         step = kernel
         if step is None:
             step = "proceed_pipelined" if "proceed_pipelined" in m.kernels else "proceed"
@@ -183,6 +192,8 @@ def main(argv=None):
     q.add_argument("--size", type=int, nargs="+", action="append", required=True)
     q.add_argument("--steps", type=int, default=50)
     q.add_argument("--kernel", default=None, help="kernel to time (default: the step)")
+    q.add_argument("--batch", type=int, default=0, help="time a batch of this many grids")
+    q.add_argument("--libs-of", default=None, help="variant whose candidates are timed")
     q.add_argument("--out", required=True)
     s = sub.add_parser("select")
     s.add_argument("path")
@@ -193,8 +204,10 @@ def main(argv=None):
     elif a.cmd == "measure":
         out = {}
         for variant, size in zip(a.variant, a.size):
-            out[variant] = {"size": size, "steps": a.steps,
-                            "results": measure(variant, size, a.steps, kernel=a.kernel)}
+            out[variant] = {"size": size if not a.batch else ["batch", a.batch],
+                            "steps": a.steps,
+                            "results": measure(variant, size, a.steps, kernel=a.kernel,
+                                               batch=a.batch, libs_of=a.libs_of)}
             with open(a.out, "w") as f:
                 json.dump(out, f, indent=1)
     else:

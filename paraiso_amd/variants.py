@@ -75,6 +75,9 @@ VARIANTS = {
     "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2, batched=True),
                   False),
     "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    # the same machine, tuned for batches of small uneven grids
+    "hydro_f32_batch": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                   batched=True), False),
     "hydro_f64": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2, batched=True),
                   True),
     "hydro_adj_f32": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,

@@ -37,10 +37,11 @@ def test_batch_equals_single_grids(require_gpu, variant, dt):
             assert np.array_equal(b.grids[g].get(f), m.get(f)), (g, f)
 
 
-def test_batch_pipelined_equals_single_grids(require_gpu):
+@pytest.mark.parametrize("variant", ["hydro_f32", "hydro_f32_batch"])
+def test_batch_pipelined_equals_single_grids(require_gpu, variant):
     """The carried-CFL step batched over uneven grids, replayed as one graph."""
     steps = 9
-    b = variants.batch("hydro_f32", SIZES)
+    b = variants.batch(variant, SIZES)
     for g, (nx, ny) in enumerate(SIZES):
         init(b.grids[g], nx, ny, 200 + g, np.float32)
     b.run("proceed_prime")
@@ -48,7 +49,7 @@ def test_batch_pipelined_equals_single_grids(require_gpu):
     b.sync()
     for g, (nx, ny) in enumerate(SIZES):
         print("batch pipelined grid %d %dx%d" % (g, nx, ny), flush=True)
-        m = variants.machine("hydro_f32", (nx, ny))
+        m = variants.machine(variant, (nx, ny))
         init(m, nx, ny, 200 + g, np.float32)
         m.run("proceed_prime")
         m.run("proceed_pipelined", steps)
