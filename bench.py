@@ -229,29 +229,32 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
             "grad_density_l1": float(np.abs(g).sum())}
 
 
-def bench_life(n=1024, gens=1000, seed=1):
-    """Conway Life 1024^2 int32 periodic, Bernoulli(0.5) cells, 1000 generations (one graph)."""
+def bench_life(n=1024, gens=1000, seed=1, per_launch=4):
+    """Conway Life 1024^2 int32 periodic, Bernoulli(0.5) cells, 1000 generations (one graph of
+    ``proceed_x4``: four generations per launch, bit-identical to four ``proceed``)."""
     import torch
     from paraiso_amd import inputs, variants
     m = variants.machine("life_i32", (n, n))
     m.run("init")
     m.set("cell", inputs.life_init((n, n), seed))
-    m.run("proceed", gens, graph=True)            # builds and warms the graph
+    kern = "proceed_x%d" % per_launch
+    m.run(kern, gens // per_launch, graph=True)      # builds and warms the graph
     m.run("init")
     m.set("cell", inputs.life_init((n, n), seed))
     m.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    m.run("proceed", gens, graph=True)
+    m.run(kern, gens // per_launch, graph=True)
     m.sync()
     t1 = time.perf_counter()
     pop, gen = int(m.get("population")), int(m.get("generation"))
-    ms = m.run_timed("proceed", 100)
+    ms = m.run_timed(kern, 25)
     cells = n * n
     return {"workload": "life-%dx%d-int32-%d-generations" % (n, n, gens), "generations": gens,
             "value": cells * gens / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
             "ms_per_step": (t1 - t0) / gens * 1e3,
-            "launch_ms_per_step": [x / 100 for x in ms], "algorithmic_bytes_per_cell": 8,
+            "launch_ms_per_launch": [x / 25 for x in ms], "generations_per_launch": per_launch,
+            "algorithmic_bytes_per_cell": 8,
             "achieved_GBps": cells * 8 * gens / (t1 - t0) / 1e9,
             "population": pop, "generation": gen}
 
@@ -281,11 +284,28 @@ def bench_wave(n=512, steps=300, every=20):
     cells = n ** 3
     dom = max(range(len(ms)), key=lambda l: ms[l])
     achieved = cells * 16 / (ms[dom] / steps * 1e-3) / 1e9
-    out = {"workload": "wave3d-%d^3-fp32" % n, "steps": steps,
-           "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
-           "ms_per_step": (t1 - t0) / steps * 1e3, "achieved_GBps": achieved,
-           "frac": achieved / HBM_PEAK_GBPS, "algorithmic_bytes_per_cell": 16,
-           "energy_drift": abs(m.get("energy") - e0) / abs(e0)}
+    drift = abs(m.get("energy") - e0) / abs(e0)
+    # two steps per launch (proceed_x2, bit-identical to two proceed calls)
+    m.set("fieldF", inputs.gaussian_pulse((n, n, n), 3, dtype=np.float32))
+    m.set("fieldG", 0.0)
+    m.run("proceed_x2", 2)
+    m.sync()
+    t2 = time.perf_counter()
+    ms2 = m.run_timed("proceed_x2", steps // 2)
+    m.sync()
+    t3 = time.perf_counter()
+    achieved2 = cells * 16 / (ms2[0] / (steps // 2) * 1e-3) / 1e9
+    runs = {"proceed": {"kernel": "proceed", "value": cells * steps / (t1 - t0) / 1e9,
+                        "ms_per_step": (t1 - t0) / steps * 1e3, "achieved_GBps": achieved,
+                        "frac": achieved / HBM_PEAK_GBPS},
+            "proceed_x2": {"kernel": "proceed_x2", "value": cells * steps / (t3 - t2) / 1e9,
+                           "ms_per_step": (t3 - t2) / steps * 1e3, "achieved_GBps": achieved2,
+                           "frac": achieved2 / HBM_PEAK_GBPS}}
+    best = max(runs.values(), key=lambda r: r["value"])   # the two store the same values
+    out = {"workload": "wave3d-%d^3-fp32" % n, "steps": steps, "unit": "Gcell-updates/s",
+           "algorithmic_bytes_per_cell_per_launch": 16, "energy_drift": drift}
+    out.update(best)
+    out["alternatives"] = runs
     # forward + adjoint of the misfit
     pulse = inputs.gaussian_pulse((n, n, n), 3, dtype=np.float32)
     target = inputs.gaussian_pulse((n, n, n), 4, dtype=np.float32)

@@ -18,6 +18,13 @@ OPEN2 = (A.Condition.Open,) * 2
 
 
 def _life():
+    """Life with ``proceed_x4``: four generations per launch (temporal blocking)."""
+    from .schemes.life import life_om
+    from .transform import unroll
+    return unroll(life_om(), "proceed", 4)
+
+
+def _life_plain():
     from .schemes.life import life_om
     return life_om()
 
@@ -62,10 +69,13 @@ def _hydro_adj(real):
 
 
 def _wave_adj(ndim, real):
+    """LinearWave with its adjoint and ``proceed_x2`` (two steps per launch)."""
     def mk():
         from .adjoint import add_adjoint
         from .schemes.linear_wave import linear_wave_om
-        return add_adjoint(linear_wave_om(ndim, real), "proceed", ["fieldF", "fieldG"])
+        from .transform import unroll
+        om = add_adjoint(linear_wave_om(ndim, real), "proceed", ["fieldF", "fieldG"])
+        return unroll(om, "proceed", 2)
     return mk
 
 
@@ -95,7 +105,7 @@ VARIANTS = {
                                                            boundary=CYC3, tile=(32, 8, 32),
                                                            march=1), True),
     # Open (non-periodic) boundaries: ghost margins, as the reference's shipped examples
-    "life_open_i32": (_life, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
+    "life_open_i32": (_life_plain, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
     "life_legacy_open_i32": (_life_legacy, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
     "hydro_legacy_open_f32": (_hydro_legacy, Setup((1024, 1024), Language.HIP, boundary=OPEN2),
                               True),

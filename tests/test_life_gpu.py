@@ -61,3 +61,21 @@ def test_life_graph_replay_matches_eager(require_gpu):
     assert np.array_equal(a.get("cell"), b.get("cell"))
     assert a.get("population") == b.get("population")
     assert a.get("generation") == b.get("generation") == 10
+
+
+@pytest.mark.parametrize("nx,ny,gens", [(1024, 1024, 1000), (100, 37, 40)])
+def test_life_four_generations_per_launch(require_gpu, nx, ny, gens):
+    """proceed_x4 (four generations per launch, halo 4) stores what 4 x proceed stores."""
+    print("life x4 nx=%d ny=%d gens=%d" % (nx, ny, gens), flush=True)
+    cells = inputs.life_init((ny, nx), 77)
+    gpu = variants.machine("life_i32", (nx, ny))
+    gpu.run("init")
+    gpu.set("cell", cells)
+    gpu.run("proceed_x4", gens // 4, graph=True)
+    ora = oracle_machine(life_om, (nx, ny))
+    ora.run("init")
+    ora.set("cell", cells)
+    ora.run("proceed", gens)
+    assert np.array_equal(gpu.get("cell"), ora.get("cell"))
+    assert gpu.get("population") == ora.get("population")
+    assert gpu.get("generation") == ora.get("generation") == gens

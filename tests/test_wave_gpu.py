@@ -74,3 +74,24 @@ def test_wave3d_adjoint_dot_product(require_gpu, variant):
     lhs.backward()
     rhs = sum((xx.detach() * xx.grad).sum() for xx in x)        # <x, A^T y> with A linear
     assert abs(float(lhs) - float(rhs)) <= 1e-12 * abs(float(lhs))
+
+
+@pytest.mark.parametrize("variant", ["wave3d_f64", "wave3d_f64_march"])
+def test_wave3d_two_steps_per_launch(require_gpu, variant):
+    """proceed_x2 (two leapfrog steps per launch, halo 2) stores what 2 x proceed stores."""
+    n = (40, 9, 70)
+    print("wave x2 %s %s" % (variant, n), flush=True)
+    rng = np.random.default_rng(8)
+    f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
+    gpu = variants.machine(variant, n)
+    ora = oracle_machine(lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
+                                             ["fieldF", "fieldG"]), n)
+    for m in (gpu, ora):
+        params(m)
+        m.set("fieldF", f0)
+        m.set("fieldG", g0)
+    gpu.run("proceed_x2", 5)
+    ora.run("proceed", 10)
+    assert np.array_equal(gpu.get("fieldF"), ora.get("fieldF"))
+    assert np.array_equal(gpu.get("fieldG"), ora.get("fieldG"))
+    assert abs(gpu.get("energy") - ora.get("energy")) <= 1e-12 * abs(ora.get("energy"))
