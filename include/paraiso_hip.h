@@ -51,7 +51,9 @@ typedef struct pr_launch_ctx {
   void* const* next;       /* per static: buffer a store writes (stored arrays) or NULL */
   char* scal;              /* device scalar block: statics, slots, counters */
   char* part;              /* this launch's reduction partials */
-  unsigned* counter;       /* this launch's arrival counter (kept at 0 between launches) */
+  unsigned* counter;       /* a zeroed device word per launch, reserved: the generated
+                              kernels finish reductions in a separate one-workgroup kernel
+                              and do not use it */
   int n[3];                /* grid extent per axis (1 for absent axes) */
   int tiles[3];            /* tiles per axis */
   unsigned ntiles;         /* product of tiles[] */
