@@ -126,3 +126,36 @@ def test_pipelined_fast_f32_matches_proceed(require_gpu):
     assert abs(a.get("time") - b.get("time")) <= 1e-6 * a.get("time")
     for f in FIELDS:
         assert rel_l1(a.get(f), b.get(f)) < 1e-6, f
+
+
+def test_fast_f32_double_sod_against_exact_solution(require_gpu):
+    """The FP32 product path on the double Sod tube (left state rho 1, p 1 on [0.25, 0.75),
+    right state rho 0.125, p 0.1 elsewhere; periodic) against the exact Riemann solution
+    (oracle/riemann_exact.py, Toro ch. 4) at t = 0.1: the same bounds the FP64 oracle meets."""
+    from oracle import riemann_exact as RE
+    nx, ny = 400, 4
+    L, R = RE.Side(1.0, 0.0, 1.0), RE.Side(0.125, 0.0, 0.1)
+    x = (np.arange(nx) + 0.5) / nx
+    hi = (x >= 0.25) & (x < 0.75)
+    rep = lambda a: np.repeat(np.asarray(a, dtype=np.float32)[None, :], ny, axis=0)
+    m = variants.machine("hydro_f32", (nx, ny))
+    for k, v in inputs.hydro_scalars(nx, ny).items():
+        m.set(k, v)
+    for name, arr in zip(FIELDS, (rep(np.where(hi, L.rho, R.rho)), rep(np.zeros(nx)),
+                                  rep(np.zeros(nx)), rep(np.where(hi, L.p, R.p)))):
+        m.set(name, arr)
+    m.run("proceed_prime")
+    steps = 0
+    while m.get("time") < 0.1 and steps < 2000:
+        m.run("proceed_pipelined")
+        steps += 1
+    t = m.get("time")
+    sub = (np.arange(nx * 8) + 0.5) / (nx * 8)
+    right = sub >= 0.5
+    r_ex, _, _ = RE.sample(L, R, 1.4, np.where(right, (sub - 0.75) / t, (0.25 - sub) / t))
+    r_ex = r_ex.reshape(nx, 8).mean(axis=1)
+    rho = m.get("density").astype(np.float64)
+    assert np.abs(rho - rho[0:1]).max() == 0.0
+    err = float(np.abs(rho[0] - r_ex).mean())
+    print("GPU FP32 double Sod, %d steps to t=%.4f: L1(rho) %.4g" % (steps, t, err))
+    assert err < 0.007
