@@ -71,17 +71,15 @@ def acc_op(op: ReduceOp, ct: CType) -> str:
     return "OmMin()" if op is ReduceOp.Min else "OmMax()"
 
 
-FAST_FP32 = {"on": False}      # set per machine by MachineEmitter
-
-
-def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType) -> str:
+def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType,
+                 fast_fp32: bool = False) -> str:
     infix = {Op.Add: "+", Op.Sub: "-", Op.Mul: "*", Op.Div: "/", Op.Mod: "%",
              Op.And: "&&", Op.Or: "||", Op.EQ: "==", Op.NE: "!=", Op.LT: "<",
              Op.LE: "<=", Op.GT: ">", Op.GE: ">="}
     t0 = arg_types[0] if arg_types else out
     if op is Op.Identity:
         return args[0]
-    if op is Op.Div and t0 is CType.Float and FAST_FP32["on"]:
+    if op is Op.Div and t0 is CType.Float and fast_fp32:
         # v_rcp_f32 (1 ulp) and a multiply instead of the scaled IEEE division
         return "%s * __builtin_amdgcn_rcpf(%s)" % (_p(args[0]), args[1])
     if op in infix:
@@ -89,7 +87,7 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
     if op is Op.Neg:
         return "-" + _p(args[0])
     if op is Op.Inv:
-        if t0 is CType.Float and FAST_FP32["on"]:
+        if t0 is CType.Float and fast_fp32:
             return "__builtin_amdgcn_rcpf(%s)" % args[0]
         return "1/" + _p(args[0])
     if op is Op.Not:
@@ -109,7 +107,7 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
     if op is Op.Signum:
         return "(%s)(((%s) > 0) - ((%s) < 0))" % (out.c_name, args[0], args[0])
     if op is Op.Sqrt:
-        if t0 is CType.Float and FAST_FP32["on"]:
+        if t0 is CType.Float and fast_fp32:
             return "__builtin_amdgcn_sqrtf(%s)" % args[0]
         return ("sqrtf(%s)" if t0 is CType.Float else "sqrt(%s)") % args[0]
     if op in (Op.Exp, Op.Log, Op.Sin, Op.Cos, Op.Tan, Op.Asin, Op.Acos, Op.Atan):
@@ -198,7 +196,7 @@ class MachineEmitter:
         if n.op == "arith":
             op, cast_to = n.payload
             return device_arith(op, ["u%d" % x for x in n.args], cast_to,
-                                [k.nodes[x].ctype for x in n.args], n.ctype)
+                                [k.nodes[x].ctype for x in n.args], n.ctype, self.fast_fp32)
         raise NotImplementedError("scalar %s" % n.op)
 
     # ------------------------------------------------------------ per launch
@@ -599,7 +597,8 @@ class MachineEmitter:
                     op, cast_to = n.payload
                     args = ["n%d_%s" % (a_, _cname(c)) for a_ in n.args]
                     lines.append("const %s %s = %s;" % (tn, name, device_arith(
-                        op, args, cast_to, [nodes[a_].ctype for a_ in n.args], n.ctype)))
+                        op, args, cast_to, [nodes[a_].ctype for a_ in n.args], n.ctype,
+                        self.fast_fp32)))
                 else:
                     raise NotImplementedError(n.op)
         return lines
@@ -641,13 +640,6 @@ class MachineEmitter:
 
     # ------------------------------------------------------------ machine
     def emit(self) -> str:
-        FAST_FP32["on"] = self.fast_fp32
-        try:
-            return self._emit()
-        finally:
-            FAST_FP32["on"] = False
-
-    def _emit(self) -> str:
         out = ["// Generated by paraiso_amd for Orthotope Machine '%s' (gfx950)." % self.name,
                "#include <math.h>", "#include <string.h>", "#define OM_NT %d" % self.nt,
                "#include \"om_device.h\"",
