@@ -23,10 +23,14 @@ int main(int argc, char** argv) {
       return 1;
     }
     std::fclose(f);
-    life.set_cell(cells);
+    // live cells through the elemental accessor, as examples/Life/main.cpp sets them
+    const int W = life.om_size_0(), H = life.om_size_1();
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x)
+        if (cells[x + W * y]) life.cell(x, y) = 1;
     const int gens = std::atoi(argv[3]);
     for (int g = 0; g < gens; ++g) life.proceed();
-    std::vector<int> out = life.cell();
+    const std::vector<int> out = life.cell();   // brought back from the device lazily
     f = std::fopen(argv[4], "wb");
     if (!f || std::fwrite(out.data(), sizeof(int), out.size(), f) != out.size()) return 1;
     std::fclose(f);

@@ -14,6 +14,29 @@ from paraiso_amd import host_example, inputs, variants
 from paraiso_amd.schemes.life import life_om
 
 
+@pytest.mark.parametrize("name", ["life_i32", "hydro_f32", "wave3d_f64", "life_legacy_open_i32",
+                                  "hydro_legacy_open_f32"])
+def test_generated_classes_compile(name, tmp_path):
+    """Every generated class (lazy host mirrors, elemental accessors with the plan's margins)
+    is valid C++ against include/paraiso_hip.h."""
+    from paraiso_amd.generator import INCLUDE_DIR
+    from paraiso_amd.generator.hip.host import emit_host_class
+    from paraiso_amd.generator.plan import translate
+    mk, setup, _ = variants.VARIANTS[name]
+    plan = translate(setup, mk())
+    hpp = emit_host_class(plan.om, setup, (plan.lower_margin, plan.upper_margin))
+    (tmp_path / "M.hpp").write_text(hpp)
+    (tmp_path / "t.cpp").write_text('#include "M.hpp"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + INCLUDE_DIR,
+                        "-I" + str(tmp_path), str(tmp_path / "t.cpp")], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr[:2000]
+    if plan.lower_margin[0]:
+        # the elemental accessor offsets local coordinates by the margins (PlanTrans.hs:140-150)
+        lm, mem = plan.lower_margin, plan.memory_size()
+        assert "(%d + i0) + %d * (%d + i1)" % (lm[0], mem[0], lm[1]) in hpp
+
+
 def test_host_example_compiles():
     exe = host_example.build()
     assert os.access(exe, os.X_OK)
