@@ -181,12 +181,12 @@ def bench_batched(n_grids=256, steps=200, seed=2024):
             "sizes_seed": seed}
 
 
-def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
+def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fast"):
     """Fixed-dt Hydro 512^2 FP32: 200 forward steps vs forward + checkpointed adjoint."""
     import numpy as np
     import torch
     from paraiso_amd import inputs, variants
-    m = variants.machine("hydro_adj_f32", (n, n))
+    m = variants.machine(variant, (n, n))
     setup_hydro(m, n, n, 7, np.float32)
     m.set("dt", dt)
     names = [s[0] for s in m.statics]
@@ -222,7 +222,8 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4):
     m.sync()
     t_ab = time.perf_counter() - t0
     g = m.get("adj_density")
-    return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "steps": steps,
+    return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "variant": variant,
+            "steps": steps,
             "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
             "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
             "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,

@@ -92,6 +92,9 @@ VARIANTS = {
                   True),
     "hydro_adj_f32": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                      tile=(32, 8)), True),
+    # FP32 bulk arithmetic with the hardware reciprocal and square root in the adjoint too
+    "hydro_adj_f32_fast": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                          tile=(32, 8)), False),
     "hydro_adj_f64": (_hydro_adj(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                       tile=(32, 8)), True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),

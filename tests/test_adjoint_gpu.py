@@ -85,11 +85,12 @@ def test_gpu_adjoint_central_differences(require_gpu):
     assert worst < 1e-5
 
 
-def test_fp32_adjoint_close_to_fp64(require_gpu):
+@pytest.mark.parametrize("variant", ["hydro_adj_f32", "hydro_adj_f32_fast"])
+def test_fp32_adjoint_close_to_fp64(require_gpu, variant):
     nx, ny, dt, nsteps, every = 32, 32, 1e-3, 8, 4
     state = generic_state(nx, ny, 9)
     _, g_ref, _ = cpu_reference_gradient(state, nx, ny, nsteps, dt)
-    m = machine("hydro_adj_f32", nx, ny, dt)
+    m = machine(variant, nx, ny, dt)
     fields = [torch.tensor(a, dtype=torch.float32, device="cuda", requires_grad=True) for a in state]
     out = rollout(m, F, fields, nsteps, every)
     _, mask = HT.kinetic_energy_in_square(*[o.detach().cpu().double() for o in out[:3]])
