@@ -72,24 +72,29 @@ def _unpack(c):
             int(c[3]) if len(c) > 3 else 0)
 
 
-def prepare(variant: str, candidates=None, jobs: int = 4) -> List[Tuple[Tuple[int, ...], int, str]]:
-    from concurrent.futures import ThreadPoolExecutor
+def _build_candidate(variant: str, c):
     from . import variants
     from .generator import build_machine
     mk, setup, precise = variants.VARIANTS[variant]
-    cands = candidates or CANDIDATES[setup.ndim]
+    tile, nt, gain, march = _unpack(c)
+    name = candidate_name(variant, tile, nt, gain, march)
+    try:
+        return c, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain, march=march),
+                                mk(), name, precise=precise)
+    except (ValueError, subprocess.CalledProcessError) as e:
+        print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
+        return c, None
 
-    def one(c):
-        tile, nt, gain, march = _unpack(c)
-        name = candidate_name(variant, tile, nt, gain, march)
-        try:
-            return (c, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain, march=march),
-                                     mk(), name, precise=precise))
-        except (ValueError, subprocess.CalledProcessError) as e:
-            print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
-            return c, None
-    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        return [r for r in ex.map(one, cands) if r[1] is not None]
+
+def prepare(variant: str, candidates=None, jobs: int = 4):
+    """Compile the candidates in ``jobs`` worker processes; returns [(candidate, library)]."""
+    from concurrent.futures import ProcessPoolExecutor
+    from . import variants
+    mk, setup, precise = variants.VARIANTS[variant]
+    cands = candidates or CANDIDATES[setup.ndim]
+    with ProcessPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_build_candidate, variant, c) for c in cands]
+        return [r for r in (f.result() for f in futs) if r[1] is not None]
 
 
 def _setup_inputs(variant: str, m, size):

@@ -132,10 +132,12 @@ def build(name: str, force: bool = False) -> str:
 
 
 def build_all(force: bool = False, jobs: int = 4) -> dict:
-    """Build every variant; hipcc runs in ``jobs`` parallel processes."""
-    from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        futs = {n: ex.submit(build, n, force) for n in VARIANTS}
+    """Build every variant in ``jobs`` worker processes (scheduling is Python, compiling is
+    hipcc; both run in parallel across variants).  The slowest variants start first."""
+    from concurrent.futures import ProcessPoolExecutor
+    order = sorted(VARIANTS, key=lambda n: ("adj" not in n, n))
+    with ProcessPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = {n: ex.submit(build, n, force) for n in order}
         return {n: f.result() for n, f in futs.items()}
 
 
