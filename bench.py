@@ -222,11 +222,27 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
     m.sync()
     t_ab = time.perf_counter() - t0
     g = m.get("adj_density")
+    m.close()
+    # the same fixed-dt forward on the plain FP64 single-grid path (correctly rounded FP64)
+    m64 = variants.machine("hydro_adj_f64", (n, n))
+    setup_hydro(m64, n, n, 7, np.float64)
+    m64.set("dt", dt)
+    m64.run("proceed", 5)
+    m64.sync()
+    setup_hydro(m64, n, n, 7, np.float64)
+    m64.set("dt", dt)
+    t0 = time.perf_counter()
+    m64.run("proceed", steps)
+    m64.sync()
+    t_fwd64 = time.perf_counter() - t0
+    m64.close()
     return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "variant": variant,
             "steps": steps,
             "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
             "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
             "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
+            "fp64_fwd_ms": t_fwd64 * 1e3,
+            "fwd_adj_over_fp64_fwd": (t_af + t_ab) / t_fwd64,
             "grad_density_l1": float(np.abs(g).sum())}
 
 

@@ -98,23 +98,6 @@ class KernelSchedule:
     scratch: List[Tuple[int, CType]] = field(default_factory=list)   # (value id, type) per scratch array
 
 
-def _cost_table(k: LKernel) -> List[int]:
-    """Number of arithmetic/shift nodes in each value's full cone (shared once)."""
-    cones: List[Set[int]] = []
-    costs = []
-    for n in k.nodes:
-        s: Set[int] = set()
-        if n.op in ("arith", "shift") and n.realm is Realm.Array:
-            s.add(n.id)
-            for a in n.args:
-                s |= cones[a]
-        elif n.op == "arith":
-            pass
-        cones.append(s if len(s) < 4000 else set(list(s)[:4000]))
-        costs.append(len(s))
-    return costs
-
-
 def _add(c, d):
     return tuple(x + y for x, y in zip(c, d))
 
@@ -223,24 +206,38 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
             readers = [out_phase if ctx == OUT else ctx for ctx in la.lc.get(v, {})]
             if any(r > q for r in readers):
                 cut.append(v)
-    if not cut:
+    if not cut and la.lds_bytes <= LDS_MAX:
         return [la]
-    # first launch: compute the cut values and store them into scratch arrays
-    first_stores = []
-    for v in cut:
-        sidx = SCRATCH_BASE + len(scratch)
-        scratch.append((v, nodes[v].ctype))
-        first_stores.append((sidx, v))
-    first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, mat_gain, tile, zero,
-                            scratch, budget, depth + 1, march)
-    # second launch: the same targets, with the cut values read back as loads
-    sidx_of = {v: s for s, v in first_stores}
-    nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of else n
-              for n in nodes]
-    k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
-    second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
-                             scratch, budget, depth + 1, march)
-    return first + second
+    n_scratch = len(scratch)
+    try:
+        if not cut:
+            raise ValueError("no value crosses the cut")
+        # first launch: compute the cut values and store them into scratch arrays
+        first_stores = []
+        for v in cut:
+            sidx = SCRATCH_BASE + len(scratch)
+            scratch.append((v, nodes[v].ctype))
+            first_stores.append((sidx, v))
+        first = _schedule_level(k, L, sorted(cut), first_stores, [], [], False, mat_gain, tile,
+                                zero, scratch, budget, depth + 1, march)
+        # second launch: the same targets, with the cut values read back as loads
+        sidx_of = {v: s for s, v in first_stores}
+        nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of
+                  else n for n in nodes]
+        k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
+        second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, mat_gain, tile,
+                                 zero, scratch, budget, depth + 1, march)
+        return first + second
+    except ValueError:
+        # a part of the cut does not fit: keep this level in one launch within the budget
+        del scratch[n_scratch:]
+        la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
+                              budget, allow_free=False, march=march)
+        la.kernel_override = k
+        if la.lds_bytes > LDS_MAX:
+            raise ValueError("a launch of kernel %s needs %d bytes of LDS at tile %s (at most %d)"
+                             % (k.name, la.lds_bytes, tuple(tile), LDS_MAX))
+        return [la]
 
 
 def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit, mat_gain,
@@ -495,6 +492,20 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if uses <= 1:
                 M.discard(v)
     phase, out_phase, members, lc, phases, lds, top = layout(M)
+    if top > LDS_MAX:
+        # the tiles the launch starts from overflow LDS (a launch behind a cut may read many
+        # scratch arrays; annotated Manifest values add theirs): read the least reused loads
+        # straight from memory, then keep the least reused annotated values in registers
+        uses = lambda v: sum(len(c) for c in lc.get(v, {}).values())
+        shed = sorted((uses(v), v) for v in M if v in loads)
+        shed += sorted((uses(v), v) for v in M if v not in loads and v not in target_set)
+        for _, v in shed:
+            M.discard(v)
+            phase, out_phase, members, lc, phases, lds, top = layout(M)
+            if SCHED_DEBUG:
+                print("schedule L%d: shed %s value %d, LDS %d" % (L, nodes[v].op, v, top))
+            if top <= LDS_MAX:
+                break
     loads_global = set(v for v in loads if v not in M)
     return ArrayLaunch(L, stores, reduces, targets, M, phases, lc, lds, top, loads_global,
                        tail, commit)

@@ -104,6 +104,10 @@ VARIANTS = {
                                     march=4), True),
     "hydro_f64_march": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                     tile=(64, 32), march=4), True),
+    # a tile whose load and annotated-value tiles alone overflow LDS: the scheduler reads the
+    # loads from memory and evaluates some annotated values in registers, checked bit for bit
+    "hydro_f64_shed": (_hydro(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                   tile=(64, 32), threads=1024), True),
     "wave3d_f64_march": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
                                                            boundary=CYC3, tile=(32, 8, 32),
                                                            march=1), True),

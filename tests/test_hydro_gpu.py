@@ -43,7 +43,8 @@ def run_pair(variant, real, nx, ny, steps, seed=11):
 @pytest.mark.parametrize("variant,real,nx,ny", [("hydro_f64", CType.Double, 64, 48),
                                                 ("hydro_f32_precise", CType.Float, 64, 48),
                                                 ("hydro_f64_march", CType.Double, 64, 48),
-                                                ("hydro_f64_march", CType.Double, 96, 64)])
+                                                ("hydro_f64_march", CType.Double, 96, 64),
+                                                ("hydro_f64_shed", CType.Double, 96, 64)])
 def test_precise_bit_exact(require_gpu, variant, real, nx, ny):
     gpu, ora = run_pair(variant, real, nx, ny, 20)
     assert gpu.get("time") == ora.get("time")
@@ -92,7 +93,8 @@ def test_uniform_state_is_fixed_point(require_gpu):
 
 
 @pytest.mark.parametrize("variant,real", [("hydro_f64", CType.Double),
-                                          ("hydro_f32_precise", CType.Float)])
+                                          ("hydro_f32_precise", CType.Float),
+                                          ("hydro_f64_shed", CType.Double)])
 def test_pipelined_step_bit_exact(require_gpu, variant, real):
     """proceed_prime + n x proceed_pipelined (the CFL minimum of the next step folded into
     this step's store phase) stores what n x proceed stores, bit for bit."""

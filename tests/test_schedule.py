@@ -1,0 +1,48 @@
+"""The HIP launch scheduler (paraiso_amd/generator/hip/schedule.py) on the CPU.
+
+A launch starts from LDS tiles of its loads and of the values the program
+annotates Manifest.  When those alone overflow the 160 KiB of LDS a workgroup
+may use (large FP64 tiles, or a launch behind a cut that reads many scratch
+arrays), the scheduler reads the least reused loads straight from memory and
+evaluates the least reused annotated values in registers instead of failing.
+The GPU tests check such a machine (``hydro_f64_shed``) bit for bit against
+the oracle."""
+from fractions import Fraction
+
+from paraiso_amd import annotation as A
+from paraiso_amd.generator.hip.ir import lower_kernel
+from paraiso_amd.generator.hip.schedule import LDS_MAX, lds_budget, schedule_kernel
+from paraiso_amd.generator.native import Language, Setup
+from paraiso_amd.generator.plan import translate
+from paraiso_amd.om.dyn import CType
+from paraiso_amd.schemes.hydro import hydro_om
+from paraiso_amd.transform import pipeline_reduce
+
+CYC2 = (A.Condition.Cyclic,) * 2
+
+
+def _pipelined_f64():
+    om = pipeline_reduce(hydro_om(CType.Double, Fraction(7, 5)), "proceed")
+    plan = translate(Setup((64, 64), Language.HIP, boundary=CYC2), om)
+    ki = [k.name for k in plan.om.kernels].index("proceed_pipelined")
+    return lower_kernel(plan.om, ki, (True, True))
+
+
+def test_overflowing_start_tiles_are_shed():
+    k = _pipelined_f64()
+    for tile, threads, gain in (((64, 64), 1024, 24), ((64, 32), 1024, 48)):
+        s = schedule_kernel(k, tile, lds_budget(threads), gain)
+        assert all(la.lds_bytes <= LDS_MAX for la in s.launches), tile
+        # the four primitive fields are read from memory, not staged
+        assert any(len(la.loads_global) == 4 for la in s.launches), tile
+
+
+def test_fitting_tiles_shed_nothing(monkeypatch, capsys):
+    """Shedding only applies when the starting tiles overflow: the FP64 tile of 32x16 cells
+    and 512 work-items sheds nothing."""
+    from paraiso_amd.generator.hip import schedule as S
+    monkeypatch.setattr(S, "SCHED_DEBUG", True)
+    k = _pipelined_f64()
+    schedule_kernel(k, (32, 16), lds_budget(512), 48)
+    out = capsys.readouterr().out
+    assert "schedule L" in out and "shed" not in out
