@@ -92,7 +92,8 @@ def profile_figures(variant, kernel):
     with open(path) as f:
         for line in f:
             for key in ("profile_config", "traffic_bytes_per_launch",
-                        "valu_wave_instructions_per_launch"):
+                        "valu_wave_instructions_per_launch", "valu_active_quad_cycles_per_launch",
+                        "valu_transcendental_per_launch"):
                 if line.startswith(key + ":"):
                     rec[key] = json.loads(line.split(":", 1)[1])
     cfg = rec.get("profile_config", {})
@@ -100,8 +101,10 @@ def profile_figures(variant, kernel):
     if (cfg.get("variant") != variant or list(setup.tile or ()) != cfg.get("tile")
             or setup.threads != cfg.get("threads")):
         return None, None
+    valu = {k: rec.get("valu_%s_per_launch" % k, {}).get(kernel)
+            for k in ("wave_instructions", "active_quad_cycles", "transcendental")}
     return (rec.get("traffic_bytes_per_launch", {}).get(kernel),
-            rec.get("valu_wave_instructions_per_launch", {}).get(kernel))
+            valu if valu["wave_instructions"] is not None else None)
 
 
 def accuracy(nx, ny, steps=500, seed=1):
@@ -462,10 +465,19 @@ def main():
         traffic, valu = profile_figures("hydro_f32", kname)
         out["roofline"]["traffic"] = traffic
         if valu is not None:
-            rate = valu / (r32["dominant_avg_ms"] * 1e-3)
-            out["valu"] = {"wave_instructions_per_launch": valu, "achieved_per_s": rate,
-                           "peak_per_s": VALU_PEAK, "frac": rate / VALU_PEAK,
-                           "source": "profiles/hydro2d_kh_2048x2048_fp32.md (SQ_INSTS_VALU)"}
+            t = r32["dominant_avg_ms"] * 1e-3
+            rate = valu["wave_instructions"] / t
+            out["valu"] = {"wave_instructions_per_launch": valu["wave_instructions"],
+                           "achieved_per_s": rate, "peak_per_s": VALU_PEAK,
+                           "frac": rate / VALU_PEAK,
+                           "source": "profiles/hydro2d_kh_2048x2048_fp32.md (SQ_INSTS_VALU, "
+                                     "SQ_ACTIVE_INST_VALU, SQ_INSTS_VALU_TRANS_F32)"}
+            if valu["active_quad_cycles"] is not None:
+                # issue slots actually used (a transcendental holds its SIMD for two), over
+                # the slots the CUs offer during the live launch time
+                out["valu"]["transcendental_per_launch"] = valu["transcendental"]
+                out["valu"]["active_quad_cycles_per_launch"] = valu["active_quad_cycles"]
+                out["valu"]["busy"] = valu["active_quad_cycles"] / (VALU_PEAK * t)
     if not args.no_fp64:
         m64, r64 = time_machine("hydro_f64", nx, ny, K, W, 1, np.float64, 64)
         out["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
