@@ -48,7 +48,9 @@ CANDIDATES = {
         ((32, 8, 64), 256, 48, 1), ((64, 8, 64), 512, 48, 1), ((64, 16, 64), 1024, 48, 1),
         ((32, 16, 64), 512, 48, 1), ((64, 8, 128), 512, 48, 2), ((32, 8, 128), 256, 48, 1),
         ((64, 8, 128), 512, 48, 4), ((64, 16, 128), 1024, 48, 2), ((128, 8, 128), 1024, 48, 2),
-        ((64, 8, 256), 512, 48, 2), ((64, 4, 128), 256, 48, 2), ((128, 4, 128), 512, 48, 2)],
+        ((64, 8, 256), 512, 48, 2), ((64, 4, 128), 256, 48, 2), ((128, 4, 128), 512, 48, 2),
+        ((64, 16, 256), 1024, 48, 2), ((64, 8, 256), 512, 48, 4), ((32, 8, 256), 256, 48, 2),
+        ((64, 16, 256), 1024, 48, 4), ((32, 8, 256), 256, 48, 4)],
 }
 
 

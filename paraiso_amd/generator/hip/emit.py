@@ -129,12 +129,15 @@ class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
                  tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
                  threads: int = NT, open_axes: Tuple[bool, ...] = (), margins=None,
-                 regions=None, march: int = 0):
+                 regions=None, march: int = 0, prefetch: bool = True):
         self.name = name
         self.nt = threads
         # march > 0: a workgroup streams along the last axis through its tile, `march`
         # planes per step, each materialised value kept as a ring of planes in LDS
         self.march = march
+        # with march: issue the global loads of step s + 1 into registers before step s
+        # computes, so their latency overlaps the step's arithmetic
+        self.prefetch = prefetch
         self._m = None
         # Open (non-periodic) axes: grids carry lower/upper margins of ghost cells; a store or
         # reduce argument computed over region [lb, n - ub) holds 0 elsewhere
@@ -389,10 +392,23 @@ class MachineEmitter:
             raise ValueError("the tile's last extent %d is not a multiple of march %d" % (Z, R))
         inner = la.phases[:-1]
         s0 = min([(ph.box[ma][0] - ph.box[ma][1]) // R for ph in inner] + [0])
+        pf = (self.prefetch and inner and inner[0].nodes
+              and all(k.nodes[v].op == "load" for v in inner[0].nodes))
         w("  // march along axis %d: %d planes per step over %d planes" % (ma, R, Z))
-        w("  for (int s = %d; s < %d; ++s) {" % (s0, Z // R))
-        for ph in inner:
+        if pf:
+            ph = inner[0]
             self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
+            self._emit_march_loads(k, la, ph, w, str(s0), declare=True)
+        w("  for (int s = %d; s < %d; ++s) {" % (s0, Z // R))
+        for pi, ph in enumerate(inner):
+            self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
+            if pf and pi == 0:
+                self._emit_march_ring_writes(k, la, ph, w)
+                w("  __syncthreads();")
+                w("  if (s + 1 < %d) {" % (Z // R))
+                self._emit_march_loads(k, la, ph, w, "(s + 1)", declare=False)
+                w("  }")
+                continue
             self._emit_phase(k, la, ph, w)
         self._m = {"R": R, "ma": ma, "lo": 0, "hi": 0}
         w("  if (s >= 0) {")
@@ -400,6 +416,94 @@ class MachineEmitter:
         w("  }")
         w("  }")
         self._m = None
+
+    def _march_box(self, ph):
+        T = self.tile
+        box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
+        W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
+        W[self._m["ma"]] = self._m["R"]
+        return box, W, W[0] * W[1] * W[2]
+
+    def _march_point(self, box, W, vol, i, step, w):
+        """Open the block of loop point p = threadIdx.x + i * NT of march step `step` (plane
+        jz of the box's leading R planes); the caller closes it with _march_close."""
+        m = self._m
+        ma = m["ma"]
+        w("  {")
+        w("    const int p = threadIdx.x + %d;" % (i * self.nt))
+        if (i + 1) * self.nt > vol:
+            w("    if (p < %d) {" % vol)
+        w("    const int l0 = p %% %d;" % W[0])
+        w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
+        w("    const int l2 = p / %d;" % (W[0] * W[1]))
+        w("    (void)l0; (void)l1; (void)l2;")
+        w("    const int jz = %s * %d + %d + l%d;" % (step, m["R"], m["hi"], ma))
+        w("    if (jz >= %d) {" % m["lo"])
+        for ax in range(3):
+            if ax == ma:
+                w("    const int g%d = X%d + jz; (void)g%d;" % (ax, ax, ax))
+            else:
+                w("    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+        return (i + 1) * self.nt > vol
+
+    def _march_close(self, guarded, w):
+        w("    }")
+        if guarded:
+            w("    }")
+        w("  }")
+
+    def _emit_march_loads(self, k, la, ph, w, step, declare):
+        """Global loads of a load-only phase for march step `step` into registers
+        pf<v>_<i> (one per loop point i of the thread), which the next ring write stores."""
+        box, W, vol = self._march_box(ph)
+        n_it = (vol + self.nt - 1) // self.nt
+        roots = list(ph.nodes)
+        if declare:
+            for v in roots:
+                tn = k.nodes[v].ctype.c_name
+                w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, i, tn)
+                                              for i in range(n_it))))
+        self._reach = [[0, 0] for _ in range(3)]
+        self._direct = False
+        bodies = [self._ctx_body(k, la, roots, box, W, False)]
+        conds = None
+        if any("om_wrap(" in ln or "om_clamp(" in ln for ln in bodies[0]):
+            reach = self._reach
+            self._direct = True
+            bodies.insert(0, self._ctx_body(k, la, roots, box, W, False))
+            self._direct = False
+            conds = []
+            T = self.tile
+            for ax in range(self.ndim):
+                lo = box[ax][0] + reach[ax][0]
+                hi = T[ax] + box[ax][1] - 1 + reach[ax][1]
+                conds.append("X%d + (%d) >= 0 && X%d + (%d) < a.n%d" % (ax, lo, ax, hi, ax))
+        w("  // prefetch: loads of phase %d for step %s" % (ph.index, step))
+        for bi, body in enumerate(bodies):
+            if conds is not None:
+                w("  if (%s) {" % " && ".join(conds) if bi == 0 else "  } else {")
+            for i in range(n_it):
+                guarded = self._march_point(box, W, vol, i, step, w)
+                for ln in body:
+                    mt = re.match(r"const \w+ n(\d+)_0(?:_0)* = (.*)$", ln)
+                    if not mt or int(mt.group(1)) not in roots:
+                        raise ValueError("prefetch expects plain loads, got: %s" % ln)
+                    w("      pf%s_%d = %s" % (mt.group(1), i, mt.group(2)))
+                self._march_close(guarded, w)
+        if conds is not None:
+            w("  }")
+
+    def _emit_march_ring_writes(self, k, la, ph, w):
+        """Store the registers loaded for this step into the value rings."""
+        box, W, vol = self._march_box(ph)
+        n_it = (vol + self.nt - 1) // self.nt
+        w("  // phase %d: values %s into their rings (loaded one step ahead)"
+          % (ph.index, ",".join("n%d" % v for v in ph.nodes)))
+        for i in range(n_it):
+            guarded = self._march_point(box, W, vol, i, "s", w)
+            for v in ph.nodes:
+                w("      s%d[%s] = pf%d_%d;" % (v, self._lds_self(la, v, W), v, i))
+            self._march_close(guarded, w)
 
     def _ring(self, la, v):
         """(depth, plane size, bias) of value v's ring of planes."""

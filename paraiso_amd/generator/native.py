@@ -39,6 +39,8 @@ class Setup:
     # > 0: each workgroup streams along the last axis through its tile (whose last extent
     # is then the streaming length), this many planes per step
     march: int = 0
+    # with march: load the next step's planes into registers while the current step computes
+    prefetch: bool = True
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)
