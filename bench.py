@@ -249,9 +249,17 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
             "grad_density_l1": float(np.abs(g).sum())}
 
 
-def bench_life(n=1024, gens=1000, seed=1, per_launch=4):
-    """Conway Life 1024^2 int32 periodic, Bernoulli(0.5) cells, 1000 generations (one graph of
-    ``proceed_x4``: four generations per launch, bit-identical to four ``proceed``)."""
+def bench_life(n=1024, gens=1000, seed=1):
+    """Conway Life 1024^2 int32 periodic, Bernoulli(0.5) cells, 1000 generations: one graph of
+    ``proceed_x4`` or ``proceed_x8`` launches (four or eight generations per launch, bit-identical
+    to as many ``proceed``); both are timed and the faster is reported."""
+    runs = [_bench_life(n, gens, seed, k) for k in (4, 8)]
+    best = dict(min(runs, key=lambda r: r["ms_per_step"]))
+    best["alternatives"] = {"proceed_x%d" % r["generations_per_launch"]: r["value"] for r in runs}
+    return best
+
+
+def _bench_life(n, gens, seed, per_launch):
     import torch
     from paraiso_amd import inputs, variants
     m = variants.machine("life_i32", (n, n))

@@ -18,10 +18,11 @@ OPEN2 = (A.Condition.Open,) * 2
 
 
 def _life():
-    """Life with ``proceed_x4``: four generations per launch (temporal blocking)."""
+    """Life with ``proceed_x4`` and ``proceed_x8``: four or eight generations per launch
+    (temporal blocking)."""
     from .schemes.life import life_om
     from .transform import unroll
-    return unroll(life_om(), "proceed", 4)
+    return unroll(unroll(life_om(), "proceed", 4), "proceed", 8)
 
 
 def _life_plain():

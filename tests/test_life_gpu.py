@@ -63,15 +63,17 @@ def test_life_graph_replay_matches_eager(require_gpu):
     assert a.get("generation") == b.get("generation") == 10
 
 
-@pytest.mark.parametrize("nx,ny,gens", [(1024, 1024, 1000), (100, 37, 40)])
-def test_life_four_generations_per_launch(require_gpu, nx, ny, gens):
-    """proceed_x4 (four generations per launch, halo 4) stores what 4 x proceed stores."""
-    print("life x4 nx=%d ny=%d gens=%d" % (nx, ny, gens), flush=True)
+@pytest.mark.parametrize("nx,ny,gens,k", [(1024, 1024, 1000, 4), (100, 37, 40, 4),
+                                          (1024, 1024, 1000, 8), (100, 37, 40, 8),
+                                          (9, 130, 24, 8)])
+def test_life_several_generations_per_launch(require_gpu, nx, ny, gens, k):
+    """proceed_x<k> (k generations per launch, halo k) stores what k x proceed stores."""
+    print("life x%d nx=%d ny=%d gens=%d" % (k, nx, ny, gens), flush=True)
     cells = inputs.life_init((ny, nx), 77)
     gpu = variants.machine("life_i32", (nx, ny))
     gpu.run("init")
     gpu.set("cell", cells)
-    gpu.run("proceed_x4", gens // 4, graph=True)
+    gpu.run("proceed_x%d" % k, gens // k, graph=True)
     ora = oracle_machine(life_om, (nx, ny))
     ora.run("init")
     ora.set("cell", cells)

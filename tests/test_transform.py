@@ -1,13 +1,14 @@
 """The OM transforms, checked on the CPU oracle (the PlanTrans restatement runs any OM):
 
 * ``pipeline_reduce``: prime + n x pipelined stores what n x proceed stores (Hydro, FP64);
-* ``unroll``: proceed_x<k> stores what k x proceed stores (Life x4, LinearWave x2);
+* ``unroll``: proceed_x<k> stores what k x proceed stores (Life x4 and x8, LinearWave x2);
 * ``add_adjoint``: the adjoint kernel of the linear wave step is the transpose of the
   step, <A x, y> = <x, A^T y>, to rounding.
 The GPU tests check the same identities on the HIP machines."""
 from fractions import Fraction
 
 import numpy as np
+import pytest
 
 from helpers import oracle_machine
 from paraiso_amd import inputs
@@ -38,19 +39,20 @@ def test_pipelined_hydro_equals_proceed():
         assert np.array_equal(a.get(f), b.get(f)), f
 
 
-def test_unrolled_life_equals_four_generations():
+@pytest.mark.parametrize("k", [4, 8])
+def test_unrolled_life_equals_k_generations(k):
     nx, ny = 30, 22
-    mk = lambda: unroll(life_om(), "proceed", 4)
+    mk = lambda: unroll(life_om(), "proceed", k)
     a, b = oracle_machine(mk, (nx, ny)), oracle_machine(mk, (nx, ny))
     cells = inputs.life_init((ny, nx), 5)
     for m in (a, b):
         m.run("init")
         m.set("cell", cells)
-    a.run("proceed", 12)
-    b.run("proceed_x4", 3)
+    a.run("proceed", 3 * k)
+    b.run("proceed_x%d" % k, 3)
     assert np.array_equal(a.get("cell"), b.get("cell"))
     assert a.get("population") == b.get("population")
-    assert a.get("generation") == b.get("generation") == 12
+    assert a.get("generation") == b.get("generation") == 3 * k
 
 
 def wave(n, real=CType.Double):
