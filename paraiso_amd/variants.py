@@ -70,7 +70,8 @@ def _hydro_adj(real):
 
 
 def _wave_adj(ndim, real):
-    """LinearWave with its adjoint and ``proceed_x2`` (two steps per launch)."""
+    """LinearWave with its adjoint and ``proceed_x2`` (two steps per launch; four steps per
+    launch need deeper halos than a streaming tile's LDS holds, and lose to two)."""
     def mk():
         from .adjoint import add_adjoint
         from .schemes.linear_wave import linear_wave_om

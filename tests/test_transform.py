@@ -1,7 +1,7 @@
 """The OM transforms, checked on the CPU oracle (the PlanTrans restatement runs any OM):
 
 * ``pipeline_reduce``: prime + n x pipelined stores what n x proceed stores (Hydro, FP64);
-* ``unroll``: proceed_x<k> stores what k x proceed stores (Life x4 and x8, LinearWave x2);
+* ``unroll``: proceed_x<k> stores what k x proceed stores (Life x4 and x8, LinearWave x2 and x4);
 * ``add_adjoint``: the adjoint kernel of the linear wave step is the transpose of the
   step, <A x, y> = <x, A^T y>, to rounding.
 The GPU tests check the same identities on the HIP machines."""
@@ -55,25 +55,26 @@ def test_unrolled_life_equals_k_generations(k):
     assert a.get("generation") == b.get("generation") == 3 * k
 
 
-def wave(n, real=CType.Double):
+def wave(n, real=CType.Double, k=2):
     m = oracle_machine(lambda: unroll(add_adjoint(linear_wave_om(3, real), "proceed",
-                                                  ["fieldF", "fieldG"]), "proceed", 2), n)
+                                                  ["fieldF", "fieldG"]), "proceed", k), n)
     m.set("c", 1.0)
     m.set("dx", 1.0)
     m.set("dt", 0.5)
     return m
 
 
-def test_unrolled_wave_equals_two_steps():
+@pytest.mark.parametrize("k", [2, 4])
+def test_unrolled_wave_equals_k_steps(k):
     n = (12, 10, 8)
     rng = np.random.default_rng(2)
     f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
-    a, b = wave(n), wave(n)
+    a, b = wave(n, k=k), wave(n, k=k)
     for m in (a, b):
         m.set("fieldF", f0)
         m.set("fieldG", g0)
-    a.run("proceed", 6)
-    b.run("proceed_x2", 3)
+    a.run("proceed", 3 * k)
+    b.run("proceed_x%d" % k, 3)
     for f in ("fieldF", "fieldG"):
         assert np.array_equal(a.get(f), b.get(f)), f
     assert a.get("energy") == b.get("energy")

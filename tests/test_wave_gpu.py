@@ -76,11 +76,11 @@ def test_wave3d_adjoint_dot_product(require_gpu, variant):
     assert abs(float(lhs) - float(rhs)) <= 1e-12 * abs(float(lhs))
 
 
-@pytest.mark.parametrize("variant", ["wave3d_f64", "wave3d_f64_march"])
-def test_wave3d_two_steps_per_launch(require_gpu, variant):
-    """proceed_x2 (two leapfrog steps per launch, halo 2) stores what 2 x proceed stores."""
+@pytest.mark.parametrize("variant,k", [("wave3d_f64", 2), ("wave3d_f64_march", 2)])
+def test_wave3d_several_steps_per_launch(require_gpu, variant, k):
+    """proceed_x<k> (k leapfrog steps per launch, halo k) stores what k x proceed stores."""
     n = (40, 9, 70)
-    print("wave x2 %s %s" % (variant, n), flush=True)
+    print("wave x%d %s %s" % (k, variant, n), flush=True)
     rng = np.random.default_rng(8)
     f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
     gpu = variants.machine(variant, n)
@@ -90,8 +90,29 @@ def test_wave3d_two_steps_per_launch(require_gpu, variant):
         params(m)
         m.set("fieldF", f0)
         m.set("fieldG", g0)
-    gpu.run("proceed_x2", 5)
-    ora.run("proceed", 10)
+    gpu.run("proceed_x%d" % k, 20 // k)
+    ora.run("proceed", 20)
     assert np.array_equal(gpu.get("fieldF"), ora.get("fieldF"))
     assert np.array_equal(gpu.get("fieldG"), ora.get("fieldG"))
     assert abs(gpu.get("energy") - ora.get("energy")) <= 1e-12 * abs(ora.get("energy"))
+
+
+@pytest.mark.parametrize("k", [2])
+def test_wave3d_f32_several_steps_per_launch_within_rounding(require_gpu, k):
+    """The FP32 build (contracted multiply-adds) at its tuned streaming tile: k steps per
+    launch against k x proceed on the same machine, equal to FP32 rounding (the two forms
+    contract different products), 20 steps of a random state."""
+    n = (128, 64, 96)
+    print("wave f32 x%d %s" % (k, n), flush=True)
+    rng = np.random.default_rng(9)
+    f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
+    a, b = variants.machine("wave3d_f32", n), variants.machine("wave3d_f32", n)
+    for m in (a, b):
+        params(m)
+        m.set("fieldF", f0)
+        m.set("fieldG", g0)
+    a.run("proceed", 20)
+    b.run("proceed_x%d" % k, 20 // k)
+    for f in ("fieldF", "fieldG"):
+        x, y = a.get(f).astype(np.float64), b.get(f).astype(np.float64)
+        assert np.abs(x - y).max() <= 1e-5 * np.abs(x).max(), f
