@@ -99,7 +99,8 @@ def profile_figures(variant, kernel):
     cfg = rec.get("profile_config", {})
     setup = variants.setup_of(variant)
     if (cfg.get("variant") != variant or list(setup.tile or ()) != cfg.get("tile")
-            or setup.threads != cfg.get("threads")):
+            or setup.threads != cfg.get("threads")
+            or bool(setup.persist) != bool(cfg.get("persist", False))):
         return None, None
     valu = {k: rec.get("valu_%s_per_launch" % k, {}).get(kernel)
             for k in ("wave_instructions", "active_quad_cycles", "transcendental")}

@@ -6,6 +6,7 @@
 //   * om_clamp       keeps halo reads of Open (non-periodic) axes inside the grid,
 //   * om_max/om_min  exactly std::max/std::min (the reference's CPU Max/Min,
 //                    Generator/PlanTrans.hs:690-691): ties and NaNs resolve as there,
+//   * om_resident_workgroups  grid size of a persistent (tile-looping) launch,
 //   * om_block_*     wave64 shuffle tree + LDS combine for Reduce: each
 //                    workgroup writes one partial, and a one-workgroup finish
 //                    kernel folds the partials in a fixed order, replacing the
@@ -14,6 +15,23 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+// Workgroups a persistent launch starts: as many as the device holds at once (per_cu per
+// CU, from the launch's LDS and work-items), a multiple of 8 so that workgroup b, which the
+// dispatcher places on XCD b % 8, keeps to that XCD's band of tiles; at most ntiles.
+static inline unsigned om_resident_workgroups(unsigned ntiles, int per_cu) {
+  static int cus = 0;
+  if (cus <= 0) {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+  }
+  unsigned n = ((unsigned)cus * (unsigned)(per_cu > 0 ? per_cu : 1)) & ~7u;
+  if (n == 0) n = 8;
+  return n < ntiles ? n : ntiles;
+}
 
 #ifndef OM_NT
 #define OM_NT 256

@@ -50,7 +50,8 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
                          fast_fp32=not precise, batched=setup.batched,
                          threads=setup.threads, open_axes=tuple(not c for c in cyclic),
                          margins=(plan.lower_margin, plan.upper_margin),
-                         regions=regions, march=setup.march, prefetch=setup.prefetch).emit()
+                         regions=regions, march=setup.march, prefetch=setup.prefetch,
+                         persist=setup.persist).emit()
     hpp = emit_host_class(om_opt, setup, (plan.lower_margin, plan.upper_margin))
     return plan, src, hpp
 

@@ -129,7 +129,7 @@ class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
                  tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
                  threads: int = NT, open_axes: Tuple[bool, ...] = (), margins=None,
-                 regions=None, march: int = 0, prefetch: bool = True):
+                 regions=None, march: int = 0, prefetch: bool = True, persist: bool = False):
         self.name = name
         self.nt = threads
         # march > 0: a workgroup streams along the last axis through its tile, `march`
@@ -138,6 +138,8 @@ class MachineEmitter:
         # with march: issue the global loads of step s + 1 into registers before step s
         # computes, so their latency overlaps the step's arithmetic
         self.prefetch = prefetch
+        # block tiles: a resident grid of workgroups loops over the tiles (see emit_launch)
+        self.persist = persist and not march
         self._m = None
         # Open (non-periodic) axes: grids carry lower/upper margins of ghost cells; a store or
         # reduce argument computed over region [lb, n - ub) holds 0 elsewhere
@@ -222,7 +224,8 @@ class MachineEmitter:
         for s in stores:
             fields.append("%s* __restrict__ st%d;" % (self.statics[s].dyn.ctype.c_name, s))
         fields += ["char* __restrict__ scal;", "char* __restrict__ part;", "unsigned* counter;",
-                   "int n0, n1, n2;", "int t0, t1, t2;", "unsigned ntiles;"]
+                   "int n0, n1, n2;", "int t0, t1, t2;", "unsigned ntiles;",
+                   "unsigned nwg;   /* reduction partials written (workgroups that reduce) */"]
         struct = "struct Args_%s {\n  %s\n};\n" % (fn, "\n  ".join(fields))
 
         decls: List[str] = []
@@ -233,13 +236,9 @@ class MachineEmitter:
             acc_types.add(acc_type(nodes[r].payload, nodes[nodes[r].args[0]].ctype))
         for at in sorted(acc_types, key=lambda c: c.value):
             decls.append("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
+        coords = self._tile_coords("tile")
         body: List[str] = []
         w = body.append
-        # tile coordinates (``tile`` is set by the single-grid or the batched prologue)
-        w("  const int X0 = (int)(tile %% (unsigned)a.t0) * %d;" % T[0])
-        w("  const int X1 = (int)((tile / (unsigned)a.t0) %% (unsigned)a.t1) * %d;" % T[1])
-        w("  const int X2 = (int)(tile / ((unsigned)a.t0 * (unsigned)a.t1)) * %d;" % T[2])
-        w("  (void)X0; (void)X1; (void)X2;")
         # uniform scalars
         for n in nodes:
             if n.realm is not Realm.Scalar:
@@ -259,21 +258,29 @@ class MachineEmitter:
             a_ct = nodes[nodes[r].args[0]].ctype
             at = acc_type(nodes[r].payload, a_ct)
             w("  %s acc%d = %s;" % (at.c_name, r, acc_init(nodes[r].payload, at)))
+        pre = list(body)             # tile-independent: scalars, LDS views, accumulators
+        work: List[str] = []
         if has_work and self.march:
-            self._emit_march(k, la, w)
+            self._emit_march(k, la, work.append)
         elif has_work:
             for ph in la.phases:
-                self._emit_phase(k, la, ph, w)
+                self._emit_phase(k, la, ph, work.append)
+        persist = has_work and self.persist
+        if persist:
+            loop = self._persistent_loop(k, la)
         # ---- epilogue: per-workgroup partials; the finish kernel does the rest
         needs_tail = bool(la.reduces) or bool(la.tail_scalars) or la.commit
-        if has_work:
+
+        def epilogue(index):
+            e = []
             for j, r in enumerate(la.reduces):
                 a_ct = nodes[nodes[r].args[0]].ctype
                 at = acc_type(nodes[r].payload, a_ct)
-                w("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
+                e.append("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
                     at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
-                w("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + tile] = br; }" % (
-                    at.c_name, j))
+                e.append("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + %s] = br; }" % (
+                    at.c_name, j, index))
+            return e if has_work else []
         fin = []
         if needs_tail:
             f = fin.append
@@ -289,7 +296,7 @@ class MachineEmitter:
             for j, r in enumerate(la.reduces):
                 a_ct = nodes[nodes[r].args[0]].ctype
                 at = acc_type(nodes[r].payload, a_ct)
-                f("  const %s fr%d = om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.ntiles, "
+                f("  const %s fr%d = om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.nwg, "
                   "(%s)%s, %s, om_scr_%s);" % (at.c_name, r, at.c_name, j, at.c_name,
                                               acc_init(nodes[r].payload, at),
                                               acc_op(nodes[r].payload, at), at.c_name))
@@ -306,12 +313,17 @@ class MachineEmitter:
         fin_decls = ["  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name)
                      for at in sorted(acc_types, key=lambda c: c.value)]
         ks = [struct]
-        core = "\n".join(body)
+        core = "\n".join(coords + pre + work + epilogue("tile"))
         ks.append("extern \"C\" __global__ void __launch_bounds__(%d) %s(Args_%s a) {" % (self.nt, fn, fn))
         ks.extend(decls)
-        ks.append("  unsigned tile = blockIdx.x;")
-        ks.append("  if ((a.ntiles & 7u) == 0u) tile = (tile & 7u) * (a.ntiles >> 3) + (tile >> 3);")
-        ks.append(core)
+        if persist:
+            ks.extend(pre)
+            ks.extend(loop)
+            ks.extend(epilogue("blockIdx.x"))
+        else:
+            ks.append("  unsigned tile = blockIdx.x;")
+            ks.append("  if ((a.ntiles & 7u) == 0u) tile = (tile & 7u) * (a.ntiles >> 3) + (tile >> 3);")
+            ks.append(core)
         ks.append("}")
         # batched form: the tiles of many grids in one launch (grid table in device memory)
         if self.batched:
@@ -352,9 +364,13 @@ class MachineEmitter:
         h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
         h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
         h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
+        if persist:
+            h.append("  a.nwg = om_resident_workgroups(c->ntiles, %d);" % self._per_cu(la))
+        else:
+            h.append("  a.nwg = c->ntiles;")
         h.append("}")
         h.append("static void pack_%s(const pr_launch_ctx* c, void* out) {" % fn)
-        h.append("  Args_%s a; fill_%s(c, a); memcpy(out, &a, sizeof a);" % (fn, fn))
+        h.append("  Args_%s a; fill_%s(c, a); a.nwg = c->ntiles; memcpy(out, &a, sizeof a);" % (fn, fn))
         h.append("}")
         h.append("static int launch_batched_%s(const pr_batch_ctx* bc) {" % fn)
         if not self.batched:
@@ -371,14 +387,145 @@ class MachineEmitter:
         h.append("static int launch_%s(const pr_launch_ctx* c) {" % fn)
         h.append("  Args_%s a; fill_%s(c, a);" % (fn, fn))
         if has_work:
-            h.append("  hipLaunchKernelGGL(%s, dim3(c->ntiles), dim3(%d), 0, (hipStream_t)c->stream, a);"
-                     % (fn, self.nt))
+            h.append("  hipLaunchKernelGGL(%s, dim3(%s), dim3(%d), 0, (hipStream_t)c->stream, a);"
+                     % (fn, "a.nwg" if persist else "c->ntiles", self.nt))
         if needs_tail:
             h.append("  hipLaunchKernelGGL(%s_fin, dim3(1), dim3(%d), 0, (hipStream_t)c->stream, a);"
                      % (fn, self.nt))
         h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"
+
+    def _tile_coords(self, tile: str) -> List[str]:
+        T = self.tile
+        return ["  const int X0 = (int)(%s %% (unsigned)a.t0) * %d;" % (tile, T[0]),
+                "  const int X1 = (int)((%s / (unsigned)a.t0) %% (unsigned)a.t1) * %d;" % (tile, T[1]),
+                "  const int X2 = (int)(%s / ((unsigned)a.t0 * (unsigned)a.t1)) * %d;" % (tile, T[2]),
+                "  (void)X0; (void)X1; (void)X2;"]
+
+    def _per_cu(self, la) -> int:
+        """Workgroups of this launch one CU holds at once, as LDS and work-items allow."""
+        per = 2048 // self.nt
+        if la.lds_bytes:
+            per = min(per, (160 * 1024) // la.lds_bytes)
+        return max(1, per)
+
+    def _persistent_loop(self, k, la) -> List[str]:
+        """Single-grid body of a resident grid: workgroup b takes tiles b, b + nwg, ... (in
+        the XCD-banded order of the one-tile-per-workgroup launch, nwg being a multiple of 8),
+        accumulating its reductions across them.  When the first phase only loads fields,
+        the loads of the next tile are issued into registers once the current tile's have
+        been written to LDS, so they are in flight while the current tile computes."""
+        out: List[str] = []
+        w = out.append
+        phases = la.phases
+        pf = (self.prefetch and len(phases) > 1 and phases[0].nodes
+              and all(k.nodes[v].op == "load" for v in phases[0].nodes))
+
+        def remap(t):
+            return ("const unsigned tile = ((a.ntiles & 7u) == 0u) ? ((%s) & 7u) * (a.ntiles >> 3) "
+                    "+ ((%s) >> 3) : (%s);" % (t, t, t))
+        if pf:
+            self._emit_tile_loads(k, la, phases[0], w, declare=True, load=False)
+            w("  if (blockIdx.x < a.ntiles) {")
+            w("  " + remap("blockIdx.x"))
+            out.extend(self._tile_coords("tile"))
+            self._emit_tile_loads(k, la, phases[0], w, declare=False)
+            w("  }")
+        w("  for (unsigned tt = blockIdx.x; tt < a.ntiles; tt += a.nwg) {")
+        w("  " + remap("tt"))
+        out.extend(self._tile_coords("tile"))
+        for pi, ph in enumerate(phases):
+            if pf and pi == 0:
+                self._emit_tile_writes(k, la, ph, w)
+                w("  __syncthreads();")
+                w("  if (tt + a.nwg < a.ntiles) {")
+                w("  " + remap("tt + a.nwg"))
+                out.extend(self._tile_coords("tile"))
+                self._emit_tile_loads(k, la, ph, w, declare=False)
+                w("  }")
+                continue
+            self._emit_phase(k, la, ph, w)
+        w("  }")
+        return out
+
+    def _emit_tile_loads(self, k, la, ph, w, declare, load=True):
+        """Global loads of a load-only phase of the tile at (X0, X1, X2) into registers
+        pf<v>_<i>, one per loop point i of the thread (``declare``: declare the registers
+        first; ``load`` False: only that)."""
+        T = self.tile
+        box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
+        W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
+        vol = W[0] * W[1] * W[2]
+        n_it = (vol + self.nt - 1) // self.nt
+        roots = list(ph.nodes)
+        if declare:
+            for v in roots:
+                tn = k.nodes[v].ctype.c_name
+                w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, i, tn) for i in range(n_it))))
+        if not load:
+            return
+        self._reach = [[0, 0] for _ in range(3)]
+        self._direct = False
+        bodies = [self._ctx_body(k, la, roots, box, W, False)]
+        conds = None
+        if any("om_wrap(" in ln or "om_clamp(" in ln for ln in bodies[0]):
+            reach = self._reach
+            self._direct = True
+            bodies.insert(0, self._ctx_body(k, la, roots, box, W, False))
+            self._direct = False
+            conds = []
+            for ax in range(self.ndim):
+                lo = box[ax][0] + reach[ax][0]
+                hi = T[ax] + box[ax][1] - 1 + reach[ax][1]
+                conds.append("X%d + (%d) >= 0 && X%d + (%d) < a.n%d" % (ax, lo, ax, hi, ax))
+        w("  // prefetch: loads of phase %d" % ph.index)
+        for bi, body in enumerate(bodies):
+            if conds is not None:
+                w("  if (%s) {" % " && ".join(conds) if bi == 0 else "  } else {")
+            for i in range(n_it):
+                guarded = self._tile_point(box, W, vol, i, w)
+                for ln in body:
+                    mt = re.match(r"const \w+ n(\d+)_0(?:_0)* = (.*)$", ln)
+                    if not mt or int(mt.group(1)) not in roots:
+                        raise ValueError("prefetch expects plain loads, got: %s" % ln)
+                    w("      pf%s_%d = %s" % (mt.group(1), i, mt.group(2)))
+                w("    }" if guarded else "  }")
+                if guarded:
+                    w("  }")
+        if conds is not None:
+            w("  }")
+
+    def _emit_tile_writes(self, k, la, ph, w):
+        T = self.tile
+        box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
+        W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
+        vol = W[0] * W[1] * W[2]
+        n_it = (vol + self.nt - 1) // self.nt
+        w("  // phase %d: values %s into LDS (loaded one tile ahead)"
+          % (ph.index, ",".join("n%d" % v for v in ph.nodes)))
+        for i in range(n_it):
+            w("  {")
+            w("    const int p = threadIdx.x + %d;" % (i * self.nt))
+            if (i + 1) * self.nt > vol:
+                w("    if (p < %d)" % vol)
+            w("    { %s }" % " ".join("s%d[p] = pf%d_%d;" % (v, v, i) for v in ph.nodes))
+            w("  }")
+
+    def _tile_point(self, box, W, vol, i, w) -> bool:
+        """Open the block of loop point p = threadIdx.x + i * NT of a block-tile phase."""
+        w("  {")
+        w("    const int p = threadIdx.x + %d;" % (i * self.nt))
+        guarded = (i + 1) * self.nt > vol
+        if guarded:
+            w("    if (p < %d) {" % vol)
+        w("    const int l0 = p %% %d;" % W[0])
+        w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
+        w("    const int l2 = p / %d;" % (W[0] * W[1]))
+        w("    (void)l0; (void)l1; (void)l2;")
+        for ax in range(3):
+            w("    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+        return guarded
 
     def _emit_march(self, k: LKernel, la: ArrayLaunch, w):
         """Stream along the last axis: at step s every phase computes the R planes at its

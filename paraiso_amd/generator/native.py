@@ -41,6 +41,9 @@ class Setup:
     march: int = 0
     # with march: load the next step's planes into registers while the current step computes
     prefetch: bool = True
+    # block tiles: launch as many workgroups as fit on the device at once and let each loop
+    # over tiles, loading the next tile's fields while it computes the current one
+    persist: bool = False
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)

@@ -86,7 +86,11 @@ VARIANTS = {
     "life_i32": (_life, Setup((64, 64), Language.HIP, boundary=CYC2), True),
     "hydro_f32": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2, batched=True),
                   False),
-    "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2), True),
+    # the headline machine's launch form (tile, work-items, resident tile loop), correctly
+    # rounded: checked bit for bit against the oracle
+    "hydro_f32_precise": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                     tile=(64, 32), threads=1024, persist=True),
+                          True),
     # the same machine, tuned for batches of small uneven grids
     "hydro_f32_batch": (_hydro(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                    batched=True), False),
@@ -128,7 +132,8 @@ def setup_of(name: str):
     rec = tuned().get(name)
     if rec is not None:
         setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)),
-                            mat_gain=int(rec.get("mat_gain", 48)), march=int(rec.get("march", 0)))
+                            mat_gain=int(rec.get("mat_gain", 48)), march=int(rec.get("march", 0)),
+                            persist=bool(rec.get("persist", setup.persist)))
     return setup
 
 
