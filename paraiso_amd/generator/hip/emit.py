@@ -451,50 +451,18 @@ class MachineEmitter:
 
     def _emit_tile_loads(self, k, la, ph, w, declare, load=True):
         """Global loads of a load-only phase of the tile at (X0, X1, X2) into registers
-        pf<v>_<i>, one per loop point i of the thread (``declare``: declare the registers
-        first; ``load`` False: only that)."""
+        pf<v>_<i>, one per loop point i of the thread."""
         T = self.tile
         box = tuple(ph.box) + ((0, 0),) * (3 - len(ph.box))
         W = [T[ax] + box[ax][1] - box[ax][0] for ax in range(3)]
         vol = W[0] * W[1] * W[2]
-        n_it = (vol + self.nt - 1) // self.nt
-        roots = list(ph.nodes)
-        if declare:
-            for v in roots:
-                tn = k.nodes[v].ctype.c_name
-                w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, i, tn) for i in range(n_it))))
-        if not load:
-            return
-        self._reach = [[0, 0] for _ in range(3)]
-        self._direct = False
-        bodies = [self._ctx_body(k, la, roots, box, W, False)]
-        conds = None
-        if any("om_wrap(" in ln or "om_clamp(" in ln for ln in bodies[0]):
-            reach = self._reach
-            self._direct = True
-            bodies.insert(0, self._ctx_body(k, la, roots, box, W, False))
-            self._direct = False
-            conds = []
-            for ax in range(self.ndim):
-                lo = box[ax][0] + reach[ax][0]
-                hi = T[ax] + box[ax][1] - 1 + reach[ax][1]
-                conds.append("X%d + (%d) >= 0 && X%d + (%d) < a.n%d" % (ax, lo, ax, hi, ax))
-        w("  // prefetch: loads of phase %d" % ph.index)
-        for bi, body in enumerate(bodies):
-            if conds is not None:
-                w("  if (%s) {" % " && ".join(conds) if bi == 0 else "  } else {")
-            for i in range(n_it):
-                guarded = self._tile_point(box, W, vol, i, w)
-                for ln in body:
-                    mt = re.match(r"const \w+ n(\d+)_0(?:_0)* = (.*)$", ln)
-                    if not mt or int(mt.group(1)) not in roots:
-                        raise ValueError("prefetch expects plain loads, got: %s" % ln)
-                    w("      pf%s_%d = %s" % (mt.group(1), i, mt.group(2)))
-                w("    }" if guarded else "  }")
-                if guarded:
-                    w("  }")
-        if conds is not None:
+
+        def close(guarded):
+            if guarded:
+                w("    }")
             w("  }")
+        self._emit_prefetch(k, la, ph, w, box, W, vol, declare, load, "",
+                            lambda i: self._tile_point(box, W, vol, i, w), close)
 
     def _emit_tile_writes(self, k, la, ph, w):
         T = self.tile
@@ -603,6 +571,16 @@ class MachineEmitter:
         """Global loads of a load-only phase for march step `step` into registers
         pf<v>_<i> (one per loop point i of the thread), which the next ring write stores."""
         box, W, vol = self._march_box(ph)
+        self._emit_prefetch(k, la, ph, w, box, W, vol, declare, True, "for step %s" % step,
+                            lambda i: self._march_point(box, W, vol, i, step, w),
+                            lambda guarded: self._march_close(guarded, w))
+
+    def _emit_prefetch(self, k, la, ph, w, box, W, vol, declare, load, what, open_point,
+                       close_point):
+        """Loads of a load-only phase into registers pf<v>_<i>, i the thread's loop point:
+        direct addressing for tiles whose reads stay inside the grid (a tile-uniform test),
+        wrapped or clamped addressing for the others.  ``declare`` declares the registers
+        first; with ``load`` False nothing else is emitted."""
         n_it = (vol + self.nt - 1) // self.nt
         roots = list(ph.nodes)
         if declare:
@@ -610,6 +588,8 @@ class MachineEmitter:
                 tn = k.nodes[v].ctype.c_name
                 w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, i, tn)
                                               for i in range(n_it))))
+        if not load:
+            return
         self._reach = [[0, 0] for _ in range(3)]
         self._direct = False
         bodies = [self._ctx_body(k, la, roots, box, W, False)]
@@ -625,18 +605,18 @@ class MachineEmitter:
                 lo = box[ax][0] + reach[ax][0]
                 hi = T[ax] + box[ax][1] - 1 + reach[ax][1]
                 conds.append("X%d + (%d) >= 0 && X%d + (%d) < a.n%d" % (ax, lo, ax, hi, ax))
-        w("  // prefetch: loads of phase %d for step %s" % (ph.index, step))
+        w("  // prefetch: loads of phase %d%s" % (ph.index, (" " + what) if what else ""))
         for bi, body in enumerate(bodies):
             if conds is not None:
                 w("  if (%s) {" % " && ".join(conds) if bi == 0 else "  } else {")
             for i in range(n_it):
-                guarded = self._march_point(box, W, vol, i, step, w)
+                guarded = open_point(i)
                 for ln in body:
                     mt = re.match(r"const \w+ n(\d+)_0(?:_0)* = (.*)$", ln)
                     if not mt or int(mt.group(1)) not in roots:
                         raise ValueError("prefetch expects plain loads, got: %s" % ln)
                     w("      pf%s_%d = %s" % (mt.group(1), i, mt.group(2)))
-                self._march_close(guarded, w)
+                close_point(guarded)
         if conds is not None:
             w("  }")
 
