@@ -269,7 +269,9 @@ class MachineEmitter:
         if persist:
             loop = self._persistent_loop(k, la)
         # ---- epilogue: per-workgroup partials; the finish kernel does the rest
-        needs_tail = bool(la.reduces) or bool(la.tail_scalars) or la.commit
+        # the finish kernel: reductions, scalars computed from them, and the commit of the
+        # kernel's scalar stores (a launch with none of these has no finish kernel)
+        needs_tail = bool(la.reduces) or bool(la.tail_scalars) or (la.commit and bool(sch.scalar_stores))
 
         def epilogue(index):
             e = []
