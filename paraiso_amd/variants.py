@@ -70,14 +70,15 @@ def _hydro_adj(real):
 
 
 def _wave_adj(ndim, real):
-    """LinearWave with its adjoint and ``proceed_x2`` (two steps per launch; four steps per
-    launch need deeper halos than a streaming tile's LDS holds, and lose to two)."""
+    """LinearWave with its adjoint, and both fused two steps per launch (``proceed_x2``,
+    ``proceed_adjoint_x2``; more steps per launch need deeper halos than a streaming tile's
+    LDS holds, and lose to two)."""
     def mk():
         from .adjoint import add_adjoint
         from .schemes.linear_wave import linear_wave_om
         from .transform import unroll
         om = add_adjoint(linear_wave_om(ndim, real), "proceed", ["fieldF", "fieldG"])
-        return unroll(om, "proceed", 2)
+        return unroll(unroll(om, "proceed", 2), "proceed_adjoint", 2)
     return mk
 
 

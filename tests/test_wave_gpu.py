@@ -76,6 +76,32 @@ def test_wave3d_adjoint_dot_product(require_gpu, variant):
     assert abs(float(lhs) - float(rhs)) <= 1e-12 * abs(float(lhs))
 
 
+@pytest.mark.parametrize("variant", ["wave3d_f64", "wave3d_f64_march"])
+def test_wave3d_two_step_adjoint_equals_two_adjoint_steps(require_gpu, variant):
+    """The rollout driven by the fused two-step kernels (proceed_x2, proceed_adjoint_x2) gives
+    the outputs and the gradient of twice as many single steps, bit for bit, and its adjoint
+    is the transpose of its forward."""
+    n = (40, 9, 70)
+    launches, every = 6, 2
+    print("wave two-step adjoint %s %s" % (variant, n), flush=True)
+    rng = np.random.default_rng(6)
+    x0 = [rng.standard_normal(n[::-1]) for _ in range(2)]
+    y = [torch.tensor(rng.standard_normal(n[::-1]), device="cuda") for _ in range(2)]
+    res = []
+    for fwd, adj, k in (("proceed", "proceed_adjoint", 1), ("proceed_x2", "proceed_adjoint_x2", 2)):
+        x = [torch.tensor(a, device="cuda", requires_grad=True) for a in x0]
+        m = variants.machine(variant, n)
+        params(m)
+        out = rollout(m, ["fieldF", "fieldG"], x, launches * 2 // k, every * 2 // k, fwd, adj)
+        lhs = sum((o * yy).sum() for o, yy in zip(out, y))
+        lhs.backward()
+        rhs = sum((xx.detach() * xx.grad).sum() for xx in x)
+        assert abs(float(lhs) - float(rhs)) <= 1e-12 * abs(float(lhs))
+        res.append([o.detach().cpu().numpy() for o in out] + [xx.grad.cpu().numpy() for xx in x])
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("variant,k", [("wave3d_f64", 2), ("wave3d_f64_march", 2)])
 def test_wave3d_several_steps_per_launch(require_gpu, variant, k):
     """proceed_x<k> (k leapfrog steps per launch, halo k) stores what k x proceed stores."""
