@@ -142,3 +142,24 @@ def test_wave3d_f32_several_steps_per_launch_within_rounding(require_gpu, k):
     for f in ("fieldF", "fieldG"):
         x, y = a.get(f).astype(np.float64), b.get(f).astype(np.float64)
         assert np.abs(x - y).max() <= 1e-5 * np.abs(x).max(), f
+
+
+def test_wave3d_f32_two_step_adjoint_within_rounding(require_gpu):
+    """The FP32 machine at its tuned streaming tile: the gradient through the two-step pair
+    (what the benchmark runs) equals the gradient through single steps to FP32 rounding."""
+    n = (128, 64, 96)
+    print("wave f32 two-step adjoint %s" % (n,), flush=True)
+    rng = np.random.default_rng(10)
+    x0 = [rng.standard_normal(n[::-1]).astype(np.float32) for _ in range(2)]
+    y = [torch.tensor(rng.standard_normal(n[::-1]).astype(np.float32), device="cuda")
+         for _ in range(2)]
+    grads = []
+    for fwd, adj, k in (("proceed", "proceed_adjoint", 1), ("proceed_x2", "proceed_adjoint_x2", 2)):
+        x = [torch.tensor(a, device="cuda", requires_grad=True) for a in x0]
+        m = variants.machine("wave3d_f32", n)
+        params(m)
+        out = rollout(m, ["fieldF", "fieldG"], x, 8 // k, 4 // k, fwd, adj)
+        sum((o * yy).sum() for o, yy in zip(out, y)).backward()
+        grads.append([xx.grad.double().cpu().numpy() for xx in x])
+    for a, b in zip(*grads):
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
