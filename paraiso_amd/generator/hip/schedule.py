@@ -36,6 +36,13 @@ CUT_GAIN = 2.0               # unlimited materialisation (with cuts) must save t
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
 LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches are cut
 LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
+RED_SCRATCH = 16 * 8          # LDS a kernel adds per Reduce: one 8-byte slot per wave (<= 16)
+
+
+def lds_cap(reduces) -> int:
+    """LDS a launch's materialised values may take: the CU's LDS less the reduction scratch
+    the kernel declares beside them."""
+    return LDS_MAX - RED_SCRATCH * len(reduces)
 
 
 def ring_depth(planes: int) -> int:
@@ -172,6 +179,8 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
     A cut after phase q stores every materialised value computed at or before
     q and read after it into a scratch array (its exact bits); the second
     launch reads those arrays as halo tiles, like loads of statics."""
+    if budget is not None:
+        budget = min(budget, lds_cap(reduces))
     la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero, budget,
                           march=march)
     la.kernel_override = k
@@ -183,9 +192,9 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
         la.kernel_override = k
         inner = [ph for ph in la.phases[:-1] if ph.index > 0]
     if la.lds_bytes <= budget or len(inner) < 2 or depth > 6:
-        if la.lds_bytes > LDS_MAX:
+        if la.lds_bytes > lds_cap(reduces):
             raise ValueError("a launch of kernel %s needs %d bytes of LDS at tile %s (at most %d)"
-                             % (k.name, la.lds_bytes, tuple(tile), LDS_MAX))
+                             % (k.name, la.lds_bytes, tuple(tile), lds_cap(reduces)))
         return [la]
     nodes = k.nodes
     # cut where half of the computed materialised values are behind us
@@ -206,7 +215,7 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
             readers = [out_phase if ctx == OUT else ctx for ctx in la.lc.get(v, {})]
             if any(r > q for r in readers):
                 cut.append(v)
-    if not cut and la.lds_bytes <= LDS_MAX:
+    if not cut and la.lds_bytes <= lds_cap(reduces):
         return [la]
     n_scratch = len(scratch)
     try:
@@ -234,9 +243,9 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
         la = _schedule_launch(k, L, targets, stores, reduces, tail, commit, mat_gain, tile, zero,
                               budget, allow_free=False, march=march)
         la.kernel_override = k
-        if la.lds_bytes > LDS_MAX:
+        if la.lds_bytes > lds_cap(reduces):
             raise ValueError("a launch of kernel %s needs %d bytes of LDS at tile %s (at most %d)"
-                             % (k.name, la.lds_bytes, tuple(tile), LDS_MAX))
+                             % (k.name, la.lds_bytes, tuple(tile), lds_cap(reduces)))
         return [la]
 
 
@@ -492,7 +501,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if uses <= 1:
                 M.discard(v)
     phase, out_phase, members, lc, phases, lds, top = layout(M)
-    if top > LDS_MAX:
+    if top > lds_cap(reduces):
         # the tiles the launch starts from overflow LDS (a launch behind a cut may read many
         # scratch arrays; annotated Manifest values add theirs): read the least reused loads
         # straight from memory, then keep the least reused annotated values in registers
@@ -504,7 +513,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             phase, out_phase, members, lc, phases, lds, top = layout(M)
             if SCHED_DEBUG:
                 print("schedule L%d: shed %s value %d, LDS %d" % (L, nodes[v].op, v, top))
-            if top <= LDS_MAX:
+            if top <= lds_cap(reduces):
                 break
     loads_global = set(v for v in loads if v not in M)
     return ArrayLaunch(L, stores, reduces, targets, M, phases, lc, lds, top, loads_global,

@@ -46,3 +46,14 @@ def test_fitting_tiles_shed_nothing(monkeypatch, capsys):
     schedule_kernel(k, (32, 16), lds_budget(512), 48)
     out = capsys.readouterr().out
     assert "schedule L" in out and "shed" not in out
+
+
+def test_reduction_scratch_is_left_free():
+    """A kernel declares one 8-byte slot per wave for each Reduce it finishes beside the
+    materialised values, and the two together must fit the CU's LDS: the FP64 step at 32x32
+    cells and 1024 work-items once filled LDS to the last byte without counting them."""
+    from paraiso_amd.generator.hip.schedule import RED_SCRATCH
+    k = _pipelined_f64()
+    s = schedule_kernel(k, (32, 32), lds_budget(1024), 48)
+    for la in s.launches:
+        assert la.lds_bytes + RED_SCRATCH * len(la.reduces) <= LDS_MAX
