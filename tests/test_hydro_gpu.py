@@ -46,7 +46,8 @@ def run_pair(variant, real, nx, ny, steps, seed=11):
                                                 ("hydro_f64_march", CType.Double, 96, 64),
                                                 ("hydro_f64_shed", CType.Double, 96, 64),
                                                 # 289 tiles: resident workgroups take two
-                                                ("hydro_f32_precise", CType.Float, 1088, 544)])
+                                                ("hydro_f32_precise", CType.Float, 1088, 544),
+                                                ("hydro_f64", CType.Double, 544, 544)])
 def test_precise_bit_exact(require_gpu, variant, real, nx, ny):
     gpu, ora = run_pair(variant, real, nx, ny, 20)
     assert gpu.get("time") == ora.get("time")
