@@ -164,3 +164,20 @@ def test_fast_f32_double_sod_against_exact_solution(require_gpu):
     err = float(np.abs(rho[0] - r_ex).mean())
     print("GPU FP32 double Sod, %d steps to t=%.4f: L1(rho) %.4g" % (steps, t, err))
     assert err < 0.007
+
+
+def test_fixed_dt_fp64_forms_agree(require_gpu):
+    """The fixed-dt FP64 step at its fastest launch form (hydro_fixed_f64: 32x32 cells, 1024
+    work-items, resident tile loop), which the adjoint's cost is compared with, stores what
+    the adjoint machine's own forward stores, bit for bit."""
+    nx, ny, steps = 160, 96, 12
+    print("fixed-dt fp64 %dx%d" % (nx, ny), flush=True)
+    a = variants.machine("hydro_fixed_f64", (nx, ny))
+    b = variants.machine("hydro_adj_f64", (nx, ny))
+    for m in (a, b):
+        setup_machine(m, nx, ny, 5, np.float64)
+        m.set("dt", 1e-4)
+        m.run("proceed", steps)
+    assert a.get("time") == b.get("time")
+    for f in FIELDS:
+        assert np.array_equal(a.get(f), b.get(f)), f
