@@ -241,6 +241,20 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
     m64.sync()
     t_fwd64 = time.perf_counter() - t0
     m64.close()
+    # and the FP32 fixed-dt step at the headline's launch form: the forward in the ratio is
+    # the adjoint machine's own, and this shows whether another form would be faster
+    m32 = variants.machine("hydro_fixed_f32", (n, n))
+    setup_hydro(m32, n, n, 7, np.float32)
+    m32.set("dt", dt)
+    m32.run("proceed", 5)
+    m32.sync()
+    setup_hydro(m32, n, n, 7, np.float32)
+    m32.set("dt", dt)
+    t0 = time.perf_counter()
+    m32.run("proceed", steps)
+    m32.sync()
+    t_fwd32 = time.perf_counter() - t0
+    m32.close()
     return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "variant": variant,
             "steps": steps,
             "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
@@ -248,6 +262,7 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
             "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
             "fp64_fwd_ms": t_fwd64 * 1e3,
             "fwd_adj_over_fp64_fwd": (t_af + t_ab) / t_fwd64,
+            "fp32_headline_form_fwd_ms": t_fwd32 * 1e3,
             "grad_density_l1": float(np.abs(g).sum())}
 
 

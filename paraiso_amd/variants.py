@@ -102,6 +102,10 @@ VARIANTS = {
     # FP32 bulk arithmetic with the hardware reciprocal and square root in the adjoint too
     "hydro_adj_f32_fast": (_hydro_adj(CType.Float), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                           tile=(32, 8)), False),
+    # the fixed-dt step in FP32 (fast arithmetic) at the headline's launch form
+    "hydro_fixed_f32": (_hydro(CType.Float, dt_mode="fixed"),
+                        Setup((64, 64), Language.HIP, boundary=CYC2, tile=(64, 32), threads=1024,
+                              persist=True), False),
     # the plain FP64 fixed-dt step at its fastest launch form (the comparison for the adjoint)
     "hydro_fixed_f64": (_hydro(CType.Double, dt_mode="fixed"),
                         Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 32), threads=1024,

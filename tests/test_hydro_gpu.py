@@ -181,3 +181,20 @@ def test_fixed_dt_fp64_forms_agree(require_gpu):
     assert a.get("time") == b.get("time")
     for f in FIELDS:
         assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_fixed_dt_fp32_forms_agree_to_rounding(require_gpu):
+    """The fixed-dt FP32 step at the headline's launch form (hydro_fixed_f32), which the
+    adjoint's cost is also compared with, against the fast adjoint machine's own forward: the
+    same arithmetic, contracted differently, so equal to FP32 rounding."""
+    nx, ny, steps = 160, 96, 12
+    print("fixed-dt fp32 %dx%d" % (nx, ny), flush=True)
+    a = variants.machine("hydro_fixed_f32", (nx, ny))
+    b = variants.machine("hydro_adj_f32_fast", (nx, ny))
+    for m in (a, b):
+        setup_machine(m, nx, ny, 5, np.float32)
+        m.set("dt", 1e-4)
+        m.run("proceed", steps)
+    for f in FIELDS:
+        x, y = a.get(f).astype(np.float64), b.get(f).astype(np.float64)
+        assert np.abs(x - y).max() <= 1e-5 * max(1.0, np.abs(x).max()), f
