@@ -81,6 +81,13 @@ def time_machine(variant, nx, ny, steps, warmup, seed, dtype, bytes_per_cell,
 VALU_PEAK = 256 * 4 * 2.4e9 / 4      # wave64 VALU instructions/s, gfx950 without packed FP32
 
 
+def build_key(variant):
+    """Hash of the generated source and compiler flags of a built machine."""
+    from paraiso_amd.generator import GEN_DIR
+    path = os.path.join(GEN_DIR, variant, "build.key")
+    return open(path).read().strip() if os.path.exists(path) else None
+
+
 def profile_figures(variant, kernel):
     """HBM bytes and VALU instructions per launch of `kernel`, from the rocprofv3 summary
     committed under profiles/ for this workload, if it was taken with the same tile."""
@@ -100,7 +107,8 @@ def profile_figures(variant, kernel):
     setup = variants.setup_of(variant)
     if (cfg.get("variant") != variant or list(setup.tile or ()) != cfg.get("tile")
             or setup.threads != cfg.get("threads")
-            or bool(setup.persist) != bool(cfg.get("persist", False))):
+            or bool(setup.persist) != bool(cfg.get("persist", False))
+            or cfg.get("build_key") != build_key(variant)):
         return None, None
     valu = {k: rec.get("valu_%s_per_launch" % k, {}).get(kernel)
             for k in ("wave_instructions", "active_quad_cycles", "transcendental")}
@@ -109,13 +117,19 @@ def profile_figures(variant, kernel):
 
 
 def accuracy(nx, ny, steps=500, seed=1):
-    """FP32 (fast) against FP64 after `steps` steps from the same KH state: relative L1 of the
-    conserved state and per field (L1 and Linf), and drift of total mass and energy."""
+    """FP32 against FP64 after `steps` steps from the same KH state: relative L1 of the
+    conserved state and per field (L1 and Linf), and drift of total mass and energy.
+
+    Two FP32 machines are compared with the FP64 one: the product machine (``hydro_f32``:
+    FP32 bulk arithmetic, constants folded exactly, compensated products by them) and the
+    reference's own FP32 program correctly rounded (``hydro_f32_precise``, which the tests
+    hold bit for bit equal to the FP32 oracle over 500 steps at 512^2)."""
     import numpy as np
     from paraiso_amd import variants
     out, totals = {}, {}
     ms = {}
-    for name, dt in (("hydro_f32", np.float32), ("hydro_f64", np.float64)):
+    for name, dt in (("hydro_f32", np.float32), ("hydro_f32_precise", np.float32),
+                     ("hydro_f64", np.float64)):
         m = variants.machine(name, (nx, ny))
         setup_hydro(m, nx, ny, seed, dt)
         c0 = conserved(m)
@@ -125,16 +139,21 @@ def accuracy(nx, ny, steps=500, seed=1):
         c = conserved(m)
         totals[name] = {"mass_drift": float(abs(c[0].sum() - c0[0].sum()) / c0[0].sum()),
                         "energy_drift": float(abs(c[3].sum() - c0[3].sum()) / c0[3].sum())}
-    a, b = ms["hydro_f32"], ms["hydro_f64"]
-    ca, cb = conserved(a), conserved(b)
+    a, r, b = ms["hydro_f32"], ms["hydro_f32_precise"], ms["hydro_f64"]
+    ca, cr, cb = conserved(a), conserved(r), conserved(b)
     out["steps"] = steps
     out["rel_l1_conserved"] = float(np.abs(ca - cb).sum() / np.abs(cb).sum())
+    out["reference_fp32_program_rel_l1_conserved"] = float(np.abs(cr - cb).sum()
+                                                           / np.abs(cb).sum())
     for f in FIELDS:
         x, y = a.get(f).astype(np.float64), b.get(f)
         out[f] = {"rel_l1": float(np.abs(x - y).sum() / np.abs(y).sum()),
                   "linf": float(np.abs(x - y).max())}
     out["time_f32"], out["time_f64"] = float(a.get("time")), float(b.get("time"))
     out["drift_f32"], out["drift_f64"] = totals["hydro_f32"], totals["hydro_f64"]
+    out["drift_reference_fp32_program"] = totals["hydro_f32_precise"]
+    for m in ms.values():
+        m.close()
     return out
 
 

@@ -36,9 +36,15 @@ def _cases():
                                     ["fieldF", "fieldG"]), (16, 12, 10), cyc3))
     out.append((lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
                                     ["fieldF", "fieldG"]), (40, 9, 70), cyc3))
-    for n in [(64, 48), (96, 64), (48, 32)]:
+    out.append((lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
+                                    ["fieldF", "fieldG"]), (64, 64, 64), cyc3))
+    out.append((lambda: add_adjoint(linear_wave_om(3, CType.Float), "proceed",
+                                    ["fieldF", "fieldG"]), (64, 64, 64), cyc3))
+    for n in [(64, 48), (96, 64), (48, 32), (80, 56), (512, 512)]:
         out.append((lambda: hydro_om(CType.Double, G), n, cyc2))
         out.append((lambda: hydro_om(CType.Float, G), n, cyc2))
+    out.append((lambda: hydro_om(CType.Float, G), (1088, 544), cyc2))
+    out.append((lambda: hydro_om(CType.Double, G), (544, 544), cyc2))
     return out
 
 

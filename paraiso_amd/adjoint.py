@@ -41,7 +41,7 @@ def _replay_value(g, v, mapped, statics):
     if isinstance(inst, Load):
         return B.load(statics[inst.static])
     if isinstance(inst, Imm):
-        return B.imm(inst.value, inst.ctype, dyn.realm)
+        return B.imm(inst.value if inst.exact is None else inst.exact, inst.ctype, dyn.realm)
     if isinstance(inst, LoadIndex):
         return B.load_index(inst.axis, dyn.ctype)
     if isinstance(inst, LoadSize):

@@ -43,7 +43,9 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
     tile = tuple(setup.tile) if setup.tile else DEFAULT_TILES[om_opt.ndim]
     cyclic = tuple(b is A.Condition.Cyclic for b in setup.boundary)
     budget = lds_budget(setup.threads)
-    lowered = [lower_kernel(om_opt, ki, cyclic) for ki in range(len(om_opt.kernels))]
+    # fast builds fold floating constants exactly and multiply by them compensated
+    lowered = [lower_kernel(om_opt, ki, cyclic, exact=not precise)
+               for ki in range(len(om_opt.kernels))]
     regions = _target_regions(plan, lowered) if not all(cyclic) else {}
     scheds = [schedule_kernel(k, tile, budget, setup.mat_gain, setup.march) for k in lowered]
     src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,

@@ -27,10 +27,13 @@ Per-point expressions follow the reference's SASA naming (``a<node>_<cursor>``,
 from __future__ import annotations
 
 import re
+from fractions import Fraction
 from typing import Dict, List, Optional, Set, Tuple
 
+import numpy as np
+
 from ...om.arith import Op, ReduceOp
-from ...om.dyn import CType, Realm
+from ...om.dyn import CType, Realm, fraction_to_float32
 from ...om.literal import c_literal
 from .ir import LKernel
 from .schedule import OUT, ArrayLaunch, KernelSchedule, ring_depth
@@ -71,8 +74,39 @@ def acc_op(op: ReduceOp, ct: CType) -> str:
     return "OmMin()" if op is ReduceOp.Min else "OmMax()"
 
 
+def _split_const(x: Fraction) -> Tuple[np.float32, np.float32]:
+    """binary32 pair (hi, lo) with hi the nearest to x and lo the nearest to x - hi."""
+    hi = fraction_to_float32(x)
+    return hi, fraction_to_float32(x - Fraction(float(hi)))
+
+
+def _const_product(x: str, c: Fraction) -> str:
+    """x * c in FP32 for an exact rational c: a plain product when c is a binary32 value,
+    else fma(x, hi, x * lo), which rounds x * c correctly but for near-ties.  A product by
+    the rounded constant instead (0.4f for 2/5, 1.4f for 7/5) is off by the constant's own
+    rounding error, the same sign in every cell and every step."""
+    hi, lo = _split_const(c)
+    if lo == 0:
+        return "%s * %s" % (_p(x), c_literal(hi, CType.Float))
+    return "__builtin_fmaf(%s, %s, %s * %s)" % (x, c_literal(hi, CType.Float), _p(x),
+                                               c_literal(lo, CType.Float))
+
+
 def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType,
-                 fast_fp32: bool = False) -> str:
+                 fast_fp32: bool = False, consts=None) -> str:
+    """C expression of one operation.  ``consts`` holds, per argument, the exact rational
+    value of a constant argument (exact-folding builds) or None."""
+    consts = consts or [None] * len(args)
+    if fast_fp32 and out is CType.Float and op in (Op.Mul, Op.Div) and any(
+            c is not None for c in consts):
+        if op is Op.Mul and consts[1] is not None and consts[0] is None:
+            return _const_product(args[0], consts[1])
+        if op is Op.Mul and consts[0] is not None and consts[1] is None:
+            return _const_product(args[1], consts[0])
+        if op is Op.Div and consts[1] is not None and consts[0] is None and consts[1] != 0:
+            return _const_product(args[0], 1 / consts[1])
+        if op is Op.Div and consts[0] is not None and consts[1] is None:
+            return _const_product("__builtin_amdgcn_rcpf(%s)" % args[1], consts[0])
     infix = {Op.Add: "+", Op.Sub: "-", Op.Mul: "*", Op.Div: "/", Op.Mod: "%",
              Op.And: "&&", Op.Or: "||", Op.EQ: "==", Op.NE: "!=", Op.LT: "<",
              Op.LE: "<=", Op.GT: ">", Op.GE: ">="}
@@ -201,7 +235,8 @@ class MachineEmitter:
         if n.op == "arith":
             op, cast_to = n.payload
             return device_arith(op, ["u%d" % x for x in n.args], cast_to,
-                                [k.nodes[x].ctype for x in n.args], n.ctype, self.fast_fp32)
+                                [k.nodes[x].ctype for x in n.args], n.ctype, self.fast_fp32,
+                                [k.exact.get(x) for x in n.args])
         raise NotImplementedError("scalar %s" % n.op)
 
     # ------------------------------------------------------------ per launch
@@ -831,7 +866,7 @@ class MachineEmitter:
                     args = ["n%d_%s" % (a_, _cname(c)) for a_ in n.args]
                     lines.append("const %s %s = %s;" % (tn, name, device_arith(
                         op, args, cast_to, [nodes[a_].ctype for a_ in n.args], n.ctype,
-                        self.fast_fp32)))
+                        self.fast_fp32, [k.exact.get(a_) for a_ in n.args])))
                 else:
                     raise NotImplementedError(n.op)
         return lines

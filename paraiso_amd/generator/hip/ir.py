@@ -14,17 +14,25 @@ kernel into a compact SSA list of value operations with:
 Folding never changes results: every rewrite is value-preserving for all grid
 sizes (IEEE operations are deterministic; comparisons are decided only when
 they hold for every ``n >= 1``).
+
+With ``exact=True`` (the fast FP32 builds) floating constants are instead
+folded in exact rational arithmetic from the literals the program wrote and
+rounded once: ``1.4 - 1`` becomes the binary32 nearest 2/5 rather than the
+difference of the rounded 1.4 and 1, and ``1 / (1.4 - 1)`` becomes exactly 2.5.
+The exact value of each floating constant is kept (``LKernel.exact``) so the
+emitter can multiply by it with a compensated product.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
+from fractions import Fraction
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
 from ... import annotation as A
 from ...om.arith import COMPARISONS, Op, ReduceOp
-from ...om.dyn import CType, Realm
+from ...om.dyn import CType, Realm, fraction_to_float32, fraction_to_float64
 from ...om.graph import (Arith, Broadcast, Graph, Imm, Load, LoadIndex, LoadSize,
                          Reduce, Shift, Store)
 
@@ -51,6 +59,8 @@ class LKernel:
     nodes: List[LNode]
     stores: List[Tuple[int, int]]           # (static idx, value id)
     ndim: int
+    # exact rational value of each floating constant (exact-folding builds only)
+    exact: Dict[int, Fraction] = field(default_factory=dict)
 
     def users(self) -> List[List[int]]:
         u = [[] for _ in self.nodes]
@@ -154,13 +164,42 @@ def _fold(op: Op, vals, ct_args: List[CType], ct_out: CType, cast_to):
     return _np_scalar(r, ct_out) if ct_out is not CType.Bool else bool(r)
 
 
+def round_exact(x: Fraction, ct: CType):
+    """The value of type ``ct`` nearest the rational ``x`` (ties to even)."""
+    return fraction_to_float32(x) if ct is CType.Float else np.float64(fraction_to_float64(x))
+
+
+def _fold_exact(op: Op, xs: List[Fraction], ct_args: List[CType], cast_to) -> Optional[Fraction]:
+    """Exact rational value of an operation on exact constants, or None."""
+    try:
+        if op is Op.Add:
+            return xs[0] + xs[1]
+        if op is Op.Sub:
+            return xs[0] - xs[1]
+        if op is Op.Mul:
+            return xs[0] * xs[1]
+        if op is Op.Div:
+            return xs[0] / xs[1]
+        if op is Op.Neg:
+            return -xs[0]
+        if op is Op.Inv:
+            return 1 / xs[0]
+        if op is Op.Cast and cast_to is not None and cast_to.is_float:
+            return xs[0]
+    except ZeroDivisionError:
+        return None
+    return None
+
+
 # ------------------------------------------------------------ building
-def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
+def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...], exact: bool = False) -> LKernel:
+    """``exact``: fold floating constants in exact rational arithmetic (see above)."""
     g: Graph = om.kernels[ki].graph
     nodes: List[LNode] = []
     memo: Dict[tuple, int] = {}
     val_map: Dict[int, int] = {}           # OM value id -> LNode id
     const: Dict[int, object] = {}
+    xval: Dict[int, Fraction] = {}         # exact value of floating (and integer) constants
     rng: Dict[int, _Rng] = {}
 
     def emit(op, args, realm, ct, payload, om_id, hint=False) -> int:
@@ -175,10 +214,19 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
         memo[key] = nid
         return nid
 
-    def make_const(value, realm, ct, om_id) -> int:
+    def make_const(value, realm, ct, om_id, ex: Optional[Fraction] = None) -> int:
         v = _np_scalar(value, ct) if ct is not CType.Bool else bool(value)
-        nid = emit("imm", (), realm, ct, v, om_id)
+        if exact and ct is not CType.Bool and ex is None:
+            ex = Fraction(int(v)) if ct is CType.Int else Fraction(float(v))
+        if exact and ct.is_float:
+            # constants that round alike but differ exactly stay apart
+            nid = emit("imm", (), realm, ct, (v, ex), om_id)
+            nodes[nid].payload = v
+        else:
+            nid = emit("imm", (), realm, ct, v, om_id)
         const[nid] = v
+        if ex is not None:
+            xval[nid] = ex
         if ct is not CType.Bool:
             rng[nid] = _Rng(None, (0, v), (0, v))
         return nid
@@ -197,7 +245,10 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
         if isinstance(inst, Load):
             nid = emit("load", (), realm, ct, inst.static, v)
         elif isinstance(inst, Imm):
-            nid = make_const(inst.value, realm, ct, v)
+            ex = None
+            if exact and ct.is_float and getattr(inst, "exact", None) is not None:
+                ex = Fraction(inst.exact)
+            nid = make_const(inst.value, realm, ct, v, ex)
         elif isinstance(inst, LoadIndex):
             nid = emit("index", (), realm, ct, inst.axis, v)
             if cyclic[inst.axis]:
@@ -207,7 +258,7 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
             rng[nid] = _Rng(inst.axis, (1, 0), (1, 0))
         elif isinstance(inst, Shift):
             if args[0] in const:
-                nid = make_const(const[args[0]], realm, ct, v)
+                nid = make_const(const[args[0]], realm, ct, v, xval.get(args[0]))
             elif all(x == 0 for x in inst.vec):
                 nid = args[0]
                 nodes[nid].om_nodes.append(v)
@@ -215,7 +266,7 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
                 nid = emit("shift", args, realm, ct, inst.vec, v)
         elif isinstance(inst, Broadcast):
             if args[0] in const:
-                nid = make_const(const[args[0]], realm, ct, v)
+                nid = make_const(const[args[0]], realm, ct, v, xval.get(args[0]))
             else:
                 nid = emit("bcast", args, realm, ct, None, v)
                 if args[0] in rng:
@@ -225,11 +276,32 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
         elif isinstance(inst, Arith):
             op = inst.op
             nid = None
-            if all(a in const for a in args):
+            if (exact and ct.is_float and args and all(a in xval for a in args)
+                    and all(nodes[a].ctype is not CType.Bool for a in args)):
+                ex = _fold_exact(op, [xval[a] for a in args], [nodes[a].ctype for a in args],
+                                 inst.cast_to)
+                if ex is not None:
+                    nid = make_const(round_exact(ex, ct), realm, ct, v, ex)
+            if nid is None and all(a in const for a in args):
                 r = _fold(op, [const[a] for a in args], [nodes[a].ctype for a in args], ct,
                           inst.cast_to)
                 if r is not None:
                     nid = make_const(r, realm, ct, v)
+            if nid is None and exact and ct.is_float and op in (Op.Add, Op.Sub, Op.Mul):
+                # x * 0, x * 1, x + 0 and x - 0 of finite operands (the sign of a zero
+                # result aside): e.g. the pressure term p * delta(i, j) of an off-diagonal
+                # momentum flux, Hydro.hs:113-114
+                x0, x1 = xval.get(args[0]), xval.get(args[1])
+                if op is Op.Mul and 0 in (x0, x1):
+                    nid = make_const(0, realm, ct, v, Fraction(0))
+                elif op is Op.Mul and (x0 == 1 or x1 == 1):
+                    nid = args[1] if x0 == 1 else args[0]
+                elif op is Op.Add and (x0 == 0 or x1 == 0):
+                    nid = args[1] if x0 == 0 else args[0]
+                elif op is Op.Sub and x1 == 0:
+                    nid = args[0]
+                if nid is not None and nid in (args[0], args[1]):
+                    nodes[nid].om_nodes.append(v)
             if nid is None and op is Op.Select and args[0] in const:
                 nid = args[1] if const[args[0]] else args[2]
                 nodes[nid].om_nodes.append(v)
@@ -264,7 +336,8 @@ def lower_kernel(om, ki: int, cyclic: Tuple[bool, ...]) -> LKernel:
         else:
             raise TypeError(inst)
         val_map[v] = nid
-    return _prune(LKernel(om.kernels[ki].name, nodes, stores, om.ndim))
+    fx = {n: x for n, x in xval.items() if nodes[n].ctype.is_float}
+    return _prune(LKernel(om.kernels[ki].name, nodes, stores, om.ndim, fx))
 
 
 def _hashable(p):
@@ -290,4 +363,5 @@ def _prune(k: LKernel) -> LKernel:
             remap[n.id] = len(out)
             out.append(LNode(len(out), n.op, tuple(remap[a] for a in n.args), n.realm, n.ctype,
                              n.payload, n.hint_manifest, n.om_nodes))
-    return LKernel(k.name, out, [(s, remap[v]) for s, v in k.stores], k.ndim)
+    return LKernel(k.name, out, [(s, remap[v]) for s, v in k.stores], k.ndim,
+                   {remap[n]: x for n, x in k.exact.items() if n in remap})

@@ -233,7 +233,7 @@ def _schedule_level(k, L, targets, stores, reduces, tail, commit, mat_gain, tile
         sidx_of = {v: s for s, v in first_stores}
         nodes2 = [LNode(n.id, "load", (), n.realm, n.ctype, sidx_of[n.id]) if n.id in sidx_of
                   else n for n in nodes]
-        k2 = LKernel(k.name, nodes2, k.stores, k.ndim)
+        k2 = LKernel(k.name, nodes2, k.stores, k.ndim, k.exact)
         second = _schedule_level(k2, L, targets, stores, reduces, tail, commit, mat_gain, tile,
                                  zero, scratch, budget, depth + 1, march)
         return first + second

@@ -542,7 +542,9 @@ def _resolve(graph: Graph) -> None:
                 lit = node.inst.literal
                 if isinstance(lit, _Pi):
                     lit = Fraction(math.pi) if c is CType.Double else Fraction(math.pi)
-                node.inst = Imm(convert_literal(lit, c), c)
+                exact = lit if c.is_float and isinstance(lit, (int, Fraction)) else None
+                node.inst = Imm(convert_literal(lit, c), c,
+                                None if exact is None else Fraction(exact))
             node.ty = None
         elif isinstance(node.inst, _PendingImm):
             raise TypeError("untyped immediate at node %d" % i)

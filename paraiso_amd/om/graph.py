@@ -78,9 +78,13 @@ class Shift:
 
 @dataclass(frozen=True)
 class Imm:
-    """An immediate; ``value`` is already converted to ``ctype`` (resolved late)."""
+    """An immediate; ``value`` is already converted to ``ctype`` (resolved late).
+
+    ``exact`` keeps the literal the program wrote (an int or a Fraction) before that
+    rounding; it takes no part in equality or in the printed form."""
     value: object
     ctype: Optional[CType] = None
+    exact: object = field(default=None, compare=False)
 
     def show(self, old: bool = False):
         if old:

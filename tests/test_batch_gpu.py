@@ -56,3 +56,29 @@ def test_batch_pipelined_equals_single_grids(require_gpu, variant):
         assert b.grids[g].get("time") == m.get("time")
         for f in F:
             assert np.array_equal(b.grids[g].get(f), m.get(f)), (g, f)
+
+
+def test_batch_config_sizes_bit_identical(require_gpu):
+    """32 grids with the benchmark's size law (sides log-uniform on [64, 1024], drawn
+    independently for nx and ny, examples/Hydro/HydroMain.hs run per grid), 20 carried-CFL
+    steps replayed as one graph: each grid equals its own single-grid run bit for bit."""
+    steps, seed = 20, 2024
+    sizes = [tuple(int(x) for x in s) for s in inputs.batched_sizes(32, seed)]
+    assert min(min(s) for s in sizes) >= 64 and max(max(s) for s in sizes) <= 1024
+    print("batch of %d grids, %d cells" % (len(sizes), sum(a * b for a, b in sizes)), flush=True)
+    b = variants.batch("hydro_f32_batch", sizes)
+    for g, (nx, ny) in enumerate(sizes):
+        init(b.grids[g], nx, ny, seed + g, np.float32)
+    b.run("proceed_prime")
+    b.run("proceed_pipelined", steps, graph=True)
+    b.sync()
+    for g, (nx, ny) in enumerate(sizes):
+        print("config-size grid %d %dx%d" % (g, nx, ny), flush=True)
+        m = variants.machine("hydro_f32_batch", (nx, ny))
+        init(m, nx, ny, seed + g, np.float32)
+        m.run("proceed_prime")
+        m.run("proceed_pipelined", steps)
+        assert b.grids[g].get("time") == m.get("time")
+        for f in F:
+            assert np.array_equal(b.grids[g].get(f), m.get(f)), (g, f)
+        m.close()

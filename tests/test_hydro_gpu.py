@@ -198,3 +198,72 @@ def test_fixed_dt_fp32_forms_agree_to_rounding(require_gpu):
     for f in FIELDS:
         x, y = a.get(f).astype(np.float64), b.get(f).astype(np.float64)
         assert np.abs(x - y).max() <= 1e-5 * max(1.0, np.abs(x).max()), f
+
+
+# ---------------------------------------------------------------- FP32 at a mid-size, long run
+KH_N, KH_STEPS, KH_SEED = 512, 500, 1
+
+
+@pytest.fixture(scope="module")
+def kh512():
+    """The FP64 and the correctly rounded FP32 oracle after 500 steps of the 512^2 KH input
+    (the headline's scheme, CFL 0.4, at a quarter of its side), shared by the tests below."""
+    out = {}
+    for real, dt in ((CType.Double, np.float64), (CType.Float, np.float32)):
+        print("oracle KH %d^2 x %d, %s" % (KH_N, KH_STEPS, real.value), flush=True)
+        ora = oracle_machine(lambda: hydro_om(real, GAMMA), (KH_N, KH_N))
+        setup_machine(ora, KH_N, KH_N, KH_SEED, dt)
+        ora.run("proceed", KH_STEPS)
+        out[real] = {f: ora.get(f) for f in FIELDS}
+        out[real]["time"] = ora.get("time")
+        ora.close()
+    return out
+
+
+def _cons(fields, gamma=1.4):
+    r, u, v, p = (np.asarray(fields[f], dtype=np.float64) for f in FIELDS)
+    return np.stack([r, r * u, r * v, p / (gamma - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def _run_headline_step(variant):
+    """The step the benchmark times: proceed_prime, then proceed_pipelined, as a graph."""
+    print("gpu KH %s %d^2 x %d" % (variant, KH_N, KH_STEPS), flush=True)
+    m = variants.machine(variant, (KH_N, KH_N))
+    setup_machine(m, KH_N, KH_N, KH_SEED, np.float32)
+    m.run("proceed_prime")
+    m.run("proceed_pipelined", KH_STEPS, graph=True)
+    out = {f: m.get(f) for f in FIELDS}
+    out["time"] = m.get("time")
+    m.close()
+    return out
+
+
+def test_precise_f32_bit_exact_long_run(require_gpu, kh512):
+    """The headline's launch form in correctly rounded FP32 (hydro_f32_precise) reproduces the
+    FP32 oracle bit for bit over 500 steps: the GPU can then stand in for the FP32 oracle at
+    the full 2048^2 size in the benchmark's accuracy report."""
+    g = _run_headline_step("hydro_f32_precise")
+    o = kh512[CType.Float]
+    assert g["time"] == o["time"]
+    for f in FIELDS:
+        assert np.array_equal(g[f], o[f]), (f, float(np.abs(g[f] - o[f]).max()))
+
+
+@pytest.mark.parametrize("variant", ["hydro_f32", "hydro_f32_batch"])
+def test_fast_f32_long_run_within_target(require_gpu, kh512, variant):
+    """The FP32 product paths (the headline machine and the batch-tuned one, single grid) after
+    500 steps against the FP64 oracle: relative L1 of the conserved state within 1e-5, and no
+    worse than the correctly rounded FP32 oracle's own error (the reference's FP32 program,
+    whose rounded constants 1.4f - 1 and 1 / (1.4f - 1) multiply to 1 + 3.6e-8 and drift the
+    pressure by that much every step)."""
+    g = _run_headline_step(variant)
+    c64 = _cons(kh512[CType.Double])
+    err_fast = rel_l1(_cons(g), c64)
+    err_ref32 = rel_l1(_cons(kh512[CType.Float]), c64)
+    print("%s: rel L1 (conserved) vs FP64 oracle %.3g; FP32 oracle %.3g" % (variant, err_fast,
+                                                                            err_ref32))
+    assert err_fast <= 1e-5
+    assert err_fast <= 1.1 * err_ref32
+    for f in ("density", "velocity0", "pressure"):
+        assert rel_l1(g[f], kh512[CType.Double][f]) <= 1e-5, f
+    assert abs(g["time"] - kh512[CType.Double]["time"]) <= 1e-5 * kh512[CType.Double]["time"]

@@ -163,3 +163,91 @@ def test_wave3d_f32_two_step_adjoint_within_rounding(require_gpu):
         grads.append([xx.grad.double().cpu().numpy() for xx in x])
     for a, b in zip(*grads):
         assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
+
+
+# ---------------------------------------------------------------- the FP32 product machine
+WN, WSTEPS = (64, 64, 64), 300
+
+
+@pytest.fixture(scope="module")
+def pulse_oracles():
+    """The FP64 and the correctly rounded FP32 oracle after 300 steps of the benchmark's input
+    (Gaussian pulse of width 8 cells from seed 3, c = 1, dx = 1, dt = 0.5) on 64^3 cells."""
+    from paraiso_amd import inputs
+    out = {}
+    for real, dt in ((CType.Double, np.float64), (CType.Float, np.float32)):
+        print("wave oracle %s %s x %d" % (real.value, WN, WSTEPS), flush=True)
+        ora = oracle_machine(lambda: add_adjoint(linear_wave_om(3, real), "proceed",
+                                                 ["fieldF", "fieldG"]), WN)
+        params(ora)
+        ora.set("fieldF", inputs.gaussian_pulse(WN, 3, dtype=dt))
+        ora.set("fieldG", 0.0)
+        ora.run("proceed", WSTEPS)
+        out[real] = (ora.get("fieldF"), ora.get("fieldG"), ora.get("energy"))
+        ora.close()
+    return out
+
+
+@pytest.mark.parametrize("kernel,k", [("proceed", 1), ("proceed_x2", 2)])
+def test_wave3d_f32_vs_f64_oracle(require_gpu, pulse_oracles, kernel, k):
+    """wave3d_f32 (the benchmark's machine, both launch forms) against the FP64 oracle after
+    300 steps: fieldF within 5e-6 relative L1 and 2e-6 absolute, fieldG (a velocity near zero
+    in most cells) within 5e-5 relative L1, the energy within 1e-6, and each no worse than
+    four times the correctly rounded FP32 oracle's own error."""
+    from helpers import rel_l1
+    from paraiso_amd import inputs
+    print("wave f32 %s vs oracle %s" % (kernel, WN), flush=True)
+    m = variants.machine("wave3d_f32", WN)
+    params(m)
+    m.set("fieldF", inputs.gaussian_pulse(WN, 3, dtype=np.float32))
+    m.set("fieldG", 0.0)
+    m.run(kernel, WSTEPS // k)
+    f64, f32 = pulse_oracles[CType.Double], pulse_oracles[CType.Float]
+    gF, gG, gE = m.get("fieldF"), m.get("fieldG"), m.get("energy")
+    eF, eG = rel_l1(gF, f64[0]), rel_l1(gG, f64[1])
+    rF, rG = rel_l1(f32[0], f64[0]), rel_l1(f32[1], f64[1])
+    print("rel L1 F %.3g (FP32 oracle %.3g), G %.3g (%.3g)" % (eF, rF, eG, rG))
+    assert eF <= 5e-6 and eF <= 4 * rF
+    assert eG <= 5e-5 and eG <= 4 * rG
+    assert np.abs(gF.astype(np.float64) - f64[0]).max() <= 2e-6
+    assert abs(gE - f64[2]) <= 1e-6 * abs(f64[2])
+
+
+def test_wave3d_f32_adjoint_central_differences(require_gpu):
+    """The FP32 gradient of the misfit J = 1/2 sum (f_N - d)^2 (300 steps through the two-step
+    kernels and their adjoint, a checkpoint every 20 steps) against central differences of
+    the FP32 forward along four seeded directions.  J is quadratic in the initial state, so
+    central differences are exact for any step: what remains is FP32 rounding."""
+    from paraiso_amd import inputs
+    n, steps, every = WN, WSTEPS, 20
+    print("wave f32 adjoint FD %s x %d" % (n, steps), flush=True)
+    f0 = inputs.gaussian_pulse(n, 3, dtype=np.float32)
+    d = inputs.gaussian_pulse(n, 4, dtype=np.float64)
+    m = variants.machine("wave3d_f32", n)
+    params(m)
+    x = [torch.tensor(f0, device="cuda", requires_grad=True),
+         torch.zeros(n[::-1], dtype=torch.float32, device="cuda", requires_grad=True)]
+    out = rollout(m, ["fieldF", "fieldG"], x, steps // 2, every // 2, "proceed_x2",
+                  "proceed_adjoint_x2")
+    dd = torch.tensor(d, device="cuda")
+    J = 0.5 * ((out[0].double() - dd) ** 2).sum()
+    J.backward()
+    gF, gG = x[0].grad.double().cpu().numpy(), x[1].grad.double().cpu().numpy()
+
+    def misfit(F0, G0):
+        m.set("fieldF", F0)
+        m.set("fieldG", G0)
+        m.run("proceed_x2", steps // 2)
+        r = m.get("fieldF").astype(np.float64) - d
+        return 0.5 * float((r * r).sum())
+    rng = np.random.default_rng(17)
+    for j in range(4):
+        dF = (0.2 * rng.standard_normal(n[::-1])).astype(np.float32)
+        dG = (0.05 * rng.standard_normal(n[::-1])).astype(np.float32)
+        eps = 0.5
+        jp = misfit(f0 + eps * dF, eps * dG)
+        jm = misfit(f0 - eps * dF, -eps * dG)
+        fd = (jp - jm) / (2 * eps)
+        ad = float((gF * dF).sum() + (gG * dG).sum())
+        print("direction %d: adjoint %.9g, central difference %.9g" % (j, ad, fd))
+        assert abs(ad - fd) <= 1e-4 * abs(fd)
