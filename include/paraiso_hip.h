@@ -192,6 +192,19 @@ int pr_snapshot_save(pr_machine* m, int slot);
 int pr_snapshot_load(pr_machine* m, int slot);
 int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every);
 int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every);
+/* The same for a step made of several kernels: the forward step runs kernels
+ * fwd[0..n_fwd) in order, and its adjoint runs adj[0..n_adj) in order.  The
+ * adjoint kernels see the state at the tape point, after fwd[0..tape_after)
+ * have run (e.g. a two-stage step split into a stage-1 and a stage-2 kernel,
+ * tape_after = 1: the adjoint sees the state and the half-step state).
+ * The replay keeps each step's state in its own snapshot slot: every kernel
+ * reads the slot of its step and writes what it stores straight into the
+ * slot that value belongs to, so the backward pass copies no state per step.
+ * pr_adjoint_backward(m, f, a, n, e) is the sequence form with one kernel
+ * each and tape_after = 0. */
+int pr_adjoint_forward_seq(pr_machine* m, const int* fwd, int n_fwd, int nsteps, int every);
+int pr_adjoint_backward_seq(pr_machine* m, const int* fwd, int n_fwd, int tape_after,
+                            const int* adj, int n_adj, int nsteps, int every);
 
 #ifdef __cplusplus
 }

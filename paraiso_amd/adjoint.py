@@ -17,7 +17,9 @@ The adjoint kernel is an ordinary OM kernel built with the Builder:
    Select/Max/Min route the cotangent to the branch the forward chose
    (``std::max``/``std::min`` tie rules), Abs by the sign, Cast by the cast back,
 4. the cotangents reaching the Loads of each differentiated static are summed
-   and stored into ``adj_<s>``.
+   and stored into ``adj_<s>``; for a static the kernel reads but does not store,
+   the cotangent it had after the kernel is added (``adj_<s>`` is loaded), unless
+   the caller names it in ``zero_in`` (known to be zero there).
 Scalars (dt, dR, ...) and Reduce results are treated as constants: the
 transform differentiates the array state only, which is what a fixed-dt step
 needs.  A Reduce feeding the differentiated state (a CFL dt) is non-smooth
@@ -67,7 +69,12 @@ def _replay_value(g, v, mapped, statics):
     raise NotImplementedError("cannot replay %r" % (inst,))
 
 
-def add_adjoint(om: OM, kernel: str, wrt: Sequence[str], name: str = None) -> OM:
+def add_adjoint(om: OM, kernel: str, wrt: Sequence[str], name: str = None,
+                zero_in: Sequence[str] = ()) -> OM:
+    """Add ``<kernel>_adjoint`` (or ``name``): one vector-Jacobian product of ``kernel``
+    with respect to the Array statics ``wrt``.  Statics in ``zero_in`` that the kernel
+    does not store have a zero cotangent after it (nothing later reads them), so their
+    adjoint is not accumulated."""
     fwd = om.kernels[om.kernel_index(kernel)].graph
     statics = list(om.setup.static_values)
     by_name = {s.name: i for i, s in enumerate(statics)}
@@ -77,8 +84,10 @@ def add_adjoint(om: OM, kernel: str, wrt: Sequence[str], name: str = None) -> OM
         if st.dyn.realm is not Realm.Array or not st.dyn.ctype.is_float:
             raise ValueError("only floating Array statics can be differentiated: %s" % s)
         adj_statics.append(B.static("adj_" + s, Realm.Array, st.dyn.ctype))
-    all_statics = statics + adj_statics
-    adj_of = {by_name[s]: adj_statics[i] for i, s in enumerate(wrt)}
+    existing = {s.name for s in statics}
+    all_statics = statics + [a for a in adj_statics if a.name not in existing]
+    adj_of = {by_name[s]: next(x for x in all_statics if x.name == "adj_" + s) for s in wrt}
+    stored = {n.inst.static for n in fwd.nodes if isinstance(n.inst, Store)}
     _check_smooth(fwd, set(adj_of))
 
     def body():
@@ -109,8 +118,11 @@ def add_adjoint(om: OM, kernel: str, wrt: Sequence[str], name: str = None) -> OM
                 continue
             _push(g, v, inst, args, ybar, mapped, cot)
         for sidx, adj in adj_of.items():
-            cs = load_cots.get(sidx)
+            cs = list(load_cots.get(sidx, []))
             st = statics[sidx]
+            if sidx not in stored and st.name not in zero_in:
+                # read, not written: the cotangent after the kernel carries through
+                cs.insert(0, B.load(adj))
             if cs:
                 B.store(adj, B.foldl1(lambda a, b: a + b, cs))
             else:

@@ -33,10 +33,6 @@ static inline unsigned om_resident_workgroups(unsigned ntiles, int per_cu) {
   return n < ntiles ? n : ntiles;
 }
 
-#ifndef OM_NT
-#define OM_NT 256
-#endif
-#define OM_NWAVES (OM_NT / 64)
 
 __device__ __forceinline__ int om_wrap(int x, int n) {
   // one conditional add/subtract covers -n <= x < 2n; tiny grids fall back to %
@@ -66,11 +62,11 @@ struct OmMax {
   template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return om_max(a, b); }
 };
 
-// Fixed-shape reduction of one value per thread to lane 0 of wave 0.
-// The xor-shuffle tree and the wave order are fixed, so results are
+// Fixed-shape reduction of one value per thread of an NT-thread workgroup; every thread
+// gets the result.  The xor-shuffle tree and the wave order are fixed, so results are
 // bit-reproducible run to run.
-template <class T, class Op>
-__device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= OM_NWAVES */) {
+template <int NT, class T, class Op>
+__device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= NT / 64 */) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -79,7 +75,7 @@ __device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= OM_NWA
   __syncthreads();
   T r = scratch[0];
 #pragma unroll
-  for (int w = 1; w < OM_NWAVES; ++w) r = op(r, scratch[w]);
+  for (int w = 1; w < NT / 64; ++w) r = op(r, scratch[w]);
   return r;   // every thread gets the block result
 }
 
@@ -87,23 +83,23 @@ __device__ __forceinline__ T om_block_reduce(T v, Op op, T* scratch /* >= OM_NWA
 // k folds partials t + (8j + k) NT, so eight independent loads are in flight per thread
 // (a one-workgroup reduction of 10^4-10^5 partials is bound by load latency); then the
 // eight are folded in order and the workgroup tree finishes.
-template <class T, class Op>
+template <int NT, class T, class Op>
 __device__ __forceinline__ T om_partials_reduce(const T* __restrict__ part, int count, T init,
                                                  Op op, T* scratch) {
   T acc[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = init;
   int i = threadIdx.x;
-  for (; i + 7 * OM_NT < count; i += 8 * OM_NT) {
+  for (; i + 7 * NT < count; i += 8 * NT) {
     T v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = part[i + k * OM_NT];
+    for (int k = 0; k < 8; ++k) v[k] = part[i + k * NT];
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[k] = op(acc[k], v[k]);
   }
-  for (int k = 0; i < count; i += OM_NT, ++k) acc[k] = op(acc[k], part[i]);
+  for (int k = 0; i < count; i += NT, ++k) acc[k] = op(acc[k], part[i]);
   T r = acc[0];
 #pragma unroll
   for (int k = 1; k < 8; ++k) r = op(r, acc[k]);
-  return om_block_reduce(r, op, scratch);
+  return om_block_reduce<NT>(r, op, scratch);
 }

@@ -25,9 +25,11 @@
 
 #include "paraiso_hip.h"
 
+// A captured graph is bound to the buffers it was captured on: the ping-pong parity of
+// every static (of every grid, for a batch) is part of its key.
 struct GraphKey {
   int kernel, count;
-  unsigned long long parity;
+  std::vector<char> parity;
   bool operator<(const GraphKey& o) const {
     if (kernel != o.kernel) return kernel < o.kernel;
     if (count != o.count) return count < o.count;
@@ -88,19 +90,27 @@ int hip_fail(pr_machine* m, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail((m), e_, #call); \
   } while (0)
 
-unsigned long long parity_mask(const pr_machine* m) {
-  unsigned long long p = 0;
-  for (size_t s = 0; s < m->cur.size() && s < 64; ++s) p |= (unsigned long long)(m->cur[s] & 1) << s;
+std::vector<char> parity_key(const pr_machine* m) {
+  std::vector<char> p(m->cur.size());
+  for (size_t s = 0; s < m->cur.size(); ++s) p[s] = (char)(m->cur[s] & 1);
   return p;
 }
 
-int launch_kernel_once(pr_machine* m, int k) {
+// One call of kernel k.  A static with a non-null entry in cur_ov is read from that buffer
+// instead of the machine's own, and a store of it goes to next_ov[s]; such statics keep
+// their ping-pong parity (the adjoint driver's tape lives outside the machine's buffers).
+int launch_kernel_ov(pr_machine* m, int k, const void* const* cur_ov, void* const* next_ov) {
   const pr_kernel_desc& kd = m->d->kernels[k];
   const int ns = m->d->n_static;
   std::vector<const void*> cur(ns, nullptr);
   std::vector<void*> next(ns, nullptr);
   for (int s = 0; s < ns; ++s) {
     if (m->d->statics[s].realm != PR_ARRAY) continue;
+    if (cur_ov && cur_ov[s]) {
+      cur[s] = cur_ov[s];
+      next[s] = next_ov ? next_ov[s] : nullptr;
+      continue;
+    }
     cur[s] = m->buf[m->cur[s]][s];
     if (m->stored_any[s]) next[s] = m->buf[1 - m->cur[s]][s];
   }
@@ -128,9 +138,14 @@ int launch_kernel_once(pr_machine* m, int k) {
                                    "]: " + hipGetErrorString(e));
     }
   }
-  for (int i = 0; i < kd.n_stored; ++i) m->cur[kd.stored[i]] ^= 1;
+  for (int i = 0; i < kd.n_stored; ++i) {
+    const int s = kd.stored[i];
+    if (!(cur_ov && cur_ov[s])) m->cur[s] ^= 1;
+  }
   return PR_OK;
 }
+
+int launch_kernel_once(pr_machine* m, int k) { return launch_kernel_ov(m, k, nullptr, nullptr); }
 
 }  // namespace
 
@@ -307,7 +322,7 @@ int pr_run_timed(pr_machine* m, int k, int count, double* launch_ms) {
 
 int pr_run_graph(pr_machine* m, int k, int count) {
   if (!m || k < 0 || k >= m->d->n_kernel || count < 1) return PR_E_INVALID;
-  GraphKey key{k, count, parity_mask(m)};
+  GraphKey key{k, count, parity_key(m)};
   auto it = m->graphs.find(key);
   if (it == m->graphs.end()) {
     std::vector<int> saved = m->cur;
@@ -501,7 +516,7 @@ __global__ void __launch_bounds__(256) pr_multi_copy(CopyBatch b) {
   for (unsigned long long i = n16 * 16 + t; i < j.n; i += stride) j.dst[i] = j.src[i];
 }
 
-int snapshot_copy(pr_machine* m, int slot, bool save) {
+int snapshot_copy(pr_machine* m, int slot, bool save, bool scalars_only = false) {
   CopyBatch b;
   int count = 0;
   unsigned long long most = 0;
@@ -521,6 +536,7 @@ int snapshot_copy(pr_machine* m, int slot, bool save) {
     char* live;
     char* keep;
     unsigned long long n;
+    if (scalars_only && m->d->statics[s].realm == PR_ARRAY) continue;
     if (m->d->statics[s].realm == PR_ARRAY) {
       live = (char*)m->buf[m->cur[s]][s];
       keep = (char*)m->snaps[slot][s];
@@ -591,57 +607,152 @@ int pr_snapshot_load(pr_machine* m, int slot) {
   return snapshot_copy(m, slot, /*save=*/false);
 }
 
-int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {
-  if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
-  if (fwd < 0 || fwd >= m->d->n_kernel)
-    return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
+int pr_adjoint_forward_seq(pr_machine* m, const int* fwd, int n_fwd, int nsteps, int every) {
+  if (!m || !fwd || n_fwd < 1 || nsteps < 0 || every < 1) return PR_E_INVALID;
+  for (int i = 0; i < n_fwd; ++i)
+    if (fwd[i] < 0 || fwd[i] >= m->d->n_kernel)
+      return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
   const int S = (nsteps + every - 1) / every;
   if ((int)m->snaps.size() < S + every)
     return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
   for (int n = 0; n < nsteps; ++n) {
     int rc;
     if (n % every == 0 && (rc = pr_snapshot_save(m, n / every))) return rc;
-    if ((rc = launch_kernel_once(m, fwd))) return rc;
+    for (int i = 0; i < n_fwd; ++i)
+      if ((rc = launch_kernel_once(m, fwd[i]))) return rc;
   }
   return PR_OK;
 }
 
-int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every) {
-  if (!m || nsteps < 0 || every < 1) return PR_E_INVALID;
-  if (fwd < 0 || fwd >= m->d->n_kernel || adj < 0 || adj >= m->d->n_kernel)
-    return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
+int pr_adjoint_forward(pr_machine* m, int fwd, int nsteps, int every) {
+  return pr_adjoint_forward_seq(m, &fwd, 1, nsteps, every);
+}
+
+namespace {
+
+bool lists(const int* xs, int n, int s) {
+  for (int i = 0; i < n; ++i)
+    if (xs[i] == s) return true;
+  return false;
+}
+
+}  // namespace
+
+int pr_adjoint_backward_seq(pr_machine* m, const int* fwd, int n_fwd, int tape_after,
+                            const int* adj, int n_adj, int nsteps, int every) {
+  if (!m || !fwd || !adj || n_fwd < 1 || n_adj < 1 || nsteps < 0 || every < 1 ||
+      tape_after < 0 || tape_after >= n_fwd)
+    return PR_E_INVALID;
+  const pr_machine_desc* d = m->d;
+  for (int i = 0; i < n_fwd; ++i)
+    if (fwd[i] < 0 || fwd[i] >= d->n_kernel)
+      return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
+  for (int i = 0; i < n_adj; ++i)
+    if (adj[i] < 0 || adj[i] >= d->n_kernel)
+      return fail(m, PR_E_INVALID, "adjoint: kernel index out of range");
   const int S = (nsteps + every - 1) / every;
   if ((int)m->snaps.size() < S + every)
     return fail(m, PR_E_INVALID, "pr_snapshot_alloc needs ceil(n/every) + every slots");
+  const int ns = d->n_static;
+  std::vector<char> snap(ns, 0), fwd_writes(ns, 0), pre_writes(ns, 0);
+  for (int s : m->snap_sel) snap[s] = 1;
+  for (int i = 0; i < n_fwd; ++i) {
+    const pr_kernel_desc& kd = d->kernels[fwd[i]];
+    for (int w = 0; w < kd.n_written; ++w) {
+      fwd_writes[kd.written[w]] = 1;
+      if (i < tape_after) pre_writes[kd.written[w]] = 1;
+    }
+  }
+  // the adjoint kernels read the state at the tape point; a linear scheme's adjoint reads
+  // none of what the forward writes, and then needs no replay
+  bool needs_state = false, needs_scalars = false;
+  for (int i = 0; i < n_adj; ++i) {
+    const pr_kernel_desc& ka = d->kernels[adj[i]];
+    for (int r = 0; r < ka.n_loaded; ++r) {
+      const int s = ka.loaded[r];
+      if (!snap[s] || !fwd_writes[s]) continue;
+      needs_state = true;
+      if (d->statics[s].realm != PR_ARRAY) needs_scalars = true;
+    }
+    for (int w = 0; w < ka.n_written; ++w)
+      if (snap[ka.written[w]])
+        return fail(m, PR_E_UNSUPPORTED, std::string("adjoint kernel ") + ka.name +
+                                             " writes a snapshotted static");
+  }
   int rc;
-  // a linear scheme's adjoint reads nothing the forward kernel writes: no replay needed
-  const pr_kernel_desc& kf = m->d->kernels[fwd];
-  const pr_kernel_desc& ka = m->d->kernels[adj];
-  bool needs_state = false;
-  for (int i = 0; i < ka.n_loaded && !needs_state; ++i)
-    for (int j = 0; j < kf.n_written && !needs_state; ++j)
-      if (ka.loaded[i] == kf.written[j] &&
-          std::find(m->snap_sel.begin(), m->snap_sel.end(), ka.loaded[i]) != m->snap_sel.end())
-        needs_state = true;
   if (!needs_state) {
     for (int j = 0; j < nsteps; ++j)
-      if ((rc = launch_kernel_once(m, adj))) return rc;
+      for (int i = 0; i < n_adj; ++i)
+        if ((rc = launch_kernel_once(m, adj[i]))) return rc;
     return S > 0 ? pr_snapshot_load(m, 0) : PR_OK;
   }
+  // A kernel before the tape point writes its arrays into the step's own tape slot, so it
+  // must not read what it overwrites there.
+  for (int i = 0; i < tape_after; ++i) {
+    const pr_kernel_desc& kd = d->kernels[fwd[i]];
+    for (int w = 0; w < kd.n_written; ++w) {
+      const int s = kd.written[w];
+      if (!snap[s] || d->statics[s].realm != PR_ARRAY) continue;
+      for (int i2 = 0; i2 <= i; ++i2) {
+        const pr_kernel_desc& k2 = d->kernels[fwd[i2]];
+        if (lists(k2.loaded, k2.n_loaded, s))
+          return fail(m, PR_E_UNSUPPORTED, std::string("kernel ") + kd.name +
+                                               " before the tape point rewrites a static it reads");
+      }
+    }
+  }
+  // Replay by rotation: the state of step j lives in tape slot j (the segment's checkpoint
+  // for j = 0, snapshot slot S + j after), each kernel reads it there and writes what it
+  // stores straight into the slot of the step the value belongs to, so no state is copied
+  // per step.  Arrays the forward never writes stay in the checkpoint.
+  std::vector<const void*> cur_ov(ns, nullptr);
+  std::vector<void*> next_ov(ns, nullptr);
+  auto slot_of = [&](int seg, int j) { return j == 0 ? seg : S + j; };
   for (int seg = S - 1; seg >= 0; --seg) {
     const int start = seg * every;
     const int len = (nsteps - start < every) ? nsteps - start : every;
-    if ((rc = pr_snapshot_load(m, seg))) return rc;
+    // scalars evolve through the replay exactly as in the forward pass
+    if ((rc = snapshot_copy(m, seg, /*save=*/false, /*scalars_only=*/true))) return rc;
     for (int j = 0; j < len; ++j) {
-      if ((rc = pr_snapshot_save(m, S + j))) return rc;
-      if (j + 1 < len && (rc = launch_kernel_once(m, fwd))) return rc;
+      std::vector<void*> here(ns, nullptr), ahead(ns, nullptr);
+      for (int s = 0; s < ns; ++s) {
+        if (!snap[s] || d->statics[s].realm != PR_ARRAY) continue;
+        here[s] = m->snaps[fwd_writes[s] ? slot_of(seg, j) : seg][s];
+        ahead[s] = (j + 1 < len) ? m->snaps[slot_of(seg, j + 1)][s] : nullptr;
+        cur_ov[s] = here[s];
+      }
+      for (int i = 0; i < n_fwd; ++i) {
+        if (i == tape_after && needs_scalars &&
+            (rc = snapshot_copy(m, S + j, /*save=*/true, /*scalars_only=*/true)))
+          return rc;
+        if (i >= tape_after && j + 1 == len) break;   // the next checkpoint holds the rest
+        const pr_kernel_desc& kd = d->kernels[fwd[i]];
+        for (int w = 0; w < kd.n_stored; ++w) {
+          const int s = kd.stored[w];
+          if (snap[s]) next_ov[s] = (i < tape_after) ? here[s] : ahead[s];
+        }
+        if ((rc = launch_kernel_ov(m, fwd[i], cur_ov.data(), next_ov.data()))) return rc;
+        for (int w = 0; w < kd.n_stored; ++w) {
+          const int s = kd.stored[w];
+          if (snap[s]) cur_ov[s] = next_ov[s];
+        }
+      }
     }
     for (int j = len - 1; j >= 0; --j) {
-      if ((rc = pr_snapshot_load(m, S + j))) return rc;
-      if ((rc = launch_kernel_once(m, adj))) return rc;
+      for (int s = 0; s < ns; ++s)
+        if (snap[s] && d->statics[s].realm == PR_ARRAY)
+          cur_ov[s] = m->snaps[fwd_writes[s] ? slot_of(seg, j) : seg][s];
+      if (needs_scalars && (rc = snapshot_copy(m, S + j, /*save=*/false, /*scalars_only=*/true)))
+        return rc;
+      for (int i = 0; i < n_adj; ++i)
+        if ((rc = launch_kernel_ov(m, adj[i], cur_ov.data(), nullptr))) return rc;
     }
   }
-  return PR_OK;
+  return S > 0 ? pr_snapshot_load(m, 0) : PR_OK;
+}
+
+int pr_adjoint_backward(pr_machine* m, int fwd, int adj, int nsteps, int every) {
+  return pr_adjoint_backward_seq(m, &fwd, 1, 0, &adj, 1, nsteps, every);
 }
 
 
@@ -657,8 +768,8 @@ struct pr_batch {
   std::vector<std::vector<int*>> tile_grid;
   std::vector<std::vector<unsigned*>> tile_base;
   std::vector<std::vector<unsigned>> total;
-  // per (kernel, launch, parity mask): packed argument blocks on the device
-  std::map<std::tuple<int, int, unsigned long long>, char*> args;
+  // per (kernel, launch, parity of every grid): packed argument blocks on the device
+  std::map<std::tuple<int, int, std::vector<char>>, char*> args;
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::string err;
 };
@@ -668,6 +779,15 @@ namespace {
 int bfail(pr_batch* b, int code, const std::string& msg) {
   if (b) b->err = msg;
   return code;
+}
+
+// Parity of every static of every grid: grids can also be run one at a time
+// (pr_batch_grid + pr_run), so their parities need not agree.
+std::vector<char> batch_parity_key(const pr_batch* b) {
+  std::vector<char> p;
+  for (const pr_machine* m : b->grids)
+    for (int c : m->cur) p.push_back((char)(c & 1));
+  return p;
 }
 
 // Build the launch context of grid machine m for launch (k, l), as launch_kernel_once does.
@@ -697,7 +817,7 @@ void grid_ctx(pr_machine* m, int k, int l, std::vector<const void*>& cur, std::v
 // Device block of packed arguments for (k, l) at the grids' current parity; built on first use
 // with synchronous copies, so a run that is captured must have built it beforehand.
 int batch_args(pr_batch* b, int k, int l, char** out) {
-  const auto key = std::make_tuple(k, l, parity_mask(b->grids[0]));
+  const auto key = std::make_tuple(k, l, batch_parity_key(b));
   auto it = b->args.find(key);
   if (it != b->args.end()) {
     *out = it->second;
@@ -859,8 +979,7 @@ int pr_batch_run_timed(pr_batch* b, int k, int count, double* launch_ms) {
 
 int pr_batch_run_graph(pr_batch* b, int k, int count) {
   if (!b || k < 0 || k >= b->d->n_kernel || count < 1) return PR_E_INVALID;
-  pr_machine* m0 = b->grids[0];
-  GraphKey key{k, count, parity_mask(m0)};
+  GraphKey key{k, count, batch_parity_key(b)};
   auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
     // build every argument block the captured sequence needs before capturing
@@ -910,3 +1029,4 @@ const char* pr_batch_last_error(const pr_batch* b) { return b ? b->err.c_str() :
 long long pr_batch_cells(const pr_batch* b) { return b ? b->cells : 0; }
 
 }  // extern "C"
+

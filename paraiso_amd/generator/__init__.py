@@ -42,18 +42,23 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
     om_opt = plan.om
     tile = tuple(setup.tile) if setup.tile else DEFAULT_TILES[om_opt.ndim]
     cyclic = tuple(b is A.Condition.Cyclic for b in setup.boundary)
-    budget = lds_budget(setup.threads)
     # fast builds fold floating constants exactly and multiply by them compensated
     lowered = [lower_kernel(om_opt, ki, cyclic, exact=not precise)
                for ki in range(len(om_opt.kernels))]
     regions = _target_regions(plan, lowered) if not all(cyclic) else {}
-    scheds = [schedule_kernel(k, tile, budget, setup.mat_gain, setup.march) for k in lowered]
+    forms, scheds = [], []
+    for ki, k in enumerate(lowered):
+        f = setup.form_of(om_opt.kernels[ki].name)
+        f["tile"] = tuple(f["tile"]) if f["tile"] else tile
+        forms.append(f)
+        scheds.append(schedule_kernel(k, f["tile"], lds_budget(f["threads"]), f["mat_gain"],
+                                      f["march"]))
     src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,
                          fast_fp32=not precise, batched=setup.batched,
                          threads=setup.threads, open_axes=tuple(not c for c in cyclic),
                          margins=(plan.lower_margin, plan.upper_margin),
                          regions=regions, march=setup.march, prefetch=setup.prefetch,
-                         persist=setup.persist).emit()
+                         persist=setup.persist, forms=forms).emit()
     hpp = emit_host_class(om_opt, setup, (plan.lower_margin, plan.upper_margin))
     return plan, src, hpp
 

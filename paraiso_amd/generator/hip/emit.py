@@ -163,7 +163,10 @@ class MachineEmitter:
     def __init__(self, name: str, statics, ndim: int, schedules: List[KernelSchedule],
                  tile: Tuple[int, ...], fast_fp32: bool = False, batched: bool = False,
                  threads: int = NT, open_axes: Tuple[bool, ...] = (), margins=None,
-                 regions=None, march: int = 0, prefetch: bool = True, persist: bool = False):
+                 regions=None, march: int = 0, prefetch: bool = True, persist: bool = False,
+                 forms: Optional[List[dict]] = None):
+        """``forms``: per kernel, the launch form overriding the machine's (keys ``tile``,
+        ``threads``, ``persist``, ``march``), or None."""
         self.name = name
         self.nt = threads
         # march > 0: a workgroup streams along the last axis through its tile, `march`
@@ -190,6 +193,9 @@ class MachineEmitter:
         self.ndim = ndim
         self.schedules = schedules
         self.tile = tuple(tile) + (1,) * (3 - len(tile))
+        self._base = {"tile": self.tile, "threads": self.nt, "persist": self.persist,
+                      "march": self.march}
+        self.forms = forms or [None] * len(schedules)
         self.ns = len(statics)
         self.n_user = len(statics)
         # scratch arrays of cut launches become internal Array statics after the user's
@@ -210,6 +216,15 @@ class MachineEmitter:
                 self.slot_index[(ki, v)] = len(self.slot_index)
 
     # ------------------------------------------------------------ helpers
+    def _use_form(self, ki: int):
+        """Tile, work-items, resident loop and streaming of kernel ki's launches."""
+        f = dict(self._base)
+        f.update(self.forms[ki] or {})
+        self.tile = tuple(f["tile"]) + (1,) * (3 - len(f["tile"]))
+        self.nt = int(f["threads"])
+        self.march = int(f["march"])
+        self.persist = bool(f["persist"]) and not self.march
+
     def fn_name(self, ki: int, launch: int) -> str:
         """Kernel symbol: machine, OM kernel and launch, so that profiles name them."""
         clean = lambda x: re.sub(r"[^A-Za-z0-9_]", "_", x)
@@ -270,7 +285,8 @@ class MachineEmitter:
         for r in la.reduces:
             acc_types.add(acc_type(nodes[r].payload, nodes[nodes[r].args[0]].ctype))
         for at in sorted(acc_types, key=lambda c: c.value):
-            decls.append("  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name))
+            decls.append("  __shared__ %s om_scr_%s[%d];" % (at.c_name, at.c_name,
+                                                             self.nt // 64))
         coords = self._tile_coords("tile")
         body: List[str] = []
         w = body.append
@@ -313,8 +329,8 @@ class MachineEmitter:
             for j, r in enumerate(la.reduces):
                 a_ct = nodes[nodes[r].args[0]].ctype
                 at = acc_type(nodes[r].payload, a_ct)
-                e.append("  { const %s br = om_block_reduce(acc%d, %s, om_scr_%s);" % (
-                    at.c_name, r, acc_op(nodes[r].payload, at), at.c_name))
+                e.append("  { const %s br = om_block_reduce<%d>(acc%d, %s, om_scr_%s);" % (
+                    at.c_name, self.nt, r, acc_op(nodes[r].payload, at), at.c_name))
                 e.append("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + %s] = br; }" % (
                     at.c_name, j, index))
             return e if has_work else []
@@ -333,8 +349,9 @@ class MachineEmitter:
             for j, r in enumerate(la.reduces):
                 a_ct = nodes[nodes[r].args[0]].ctype
                 at = acc_type(nodes[r].payload, a_ct)
-                f("  const %s fr%d = om_partials_reduce((const %s*)a.part + %d * a.ntiles, (int)a.nwg, "
-                  "(%s)%s, %s, om_scr_%s);" % (at.c_name, r, at.c_name, j, at.c_name,
+                f("  const %s fr%d = om_partials_reduce<%d>((const %s*)a.part + %d * a.ntiles, "
+                  "(int)a.nwg, (%s)%s, %s, om_scr_%s);" % (at.c_name, r, self.nt, at.c_name, j,
+                                                         at.c_name,
                                               acc_init(nodes[r].payload, at),
                                               acc_op(nodes[r].payload, at), at.c_name))
             f("  if (threadIdx.x == 0) {")
@@ -347,7 +364,7 @@ class MachineEmitter:
                 for s_, v in sch.scalar_stores:
                     f("    *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s_, v))
             f("  }")
-        fin_decls = ["  __shared__ %s om_scr_%s[OM_NWAVES];" % (at.c_name, at.c_name)
+        fin_decls = ["  __shared__ %s om_scr_%s[%d];" % (at.c_name, at.c_name, self.nt // 64)
                      for at in sorted(acc_types, key=lambda c: c.value)]
         ks = [struct]
         core = "\n".join(coords + pre + work + epilogue("tile"))
@@ -909,13 +926,14 @@ class MachineEmitter:
     # ------------------------------------------------------------ machine
     def emit(self) -> str:
         out = ["// Generated by paraiso_amd for Orthotope Machine '%s' (gfx950)." % self.name,
-               "#include <math.h>", "#include <string.h>", "#define OM_NT %d" % self.nt,
+               "#include <math.h>", "#include <string.h>",
                "#include \"om_device.h\"",
                "#include \"paraiso_hip.h\"", ""]
         launchers = []
         kdescs = []
         for ki, sch in enumerate(self.schedules):
             ldesc = []
+            self._use_form(ki)
             for la in sch.launches:
                 ksrc, hsrc = self.emit_launch(ki, sch, la)
                 out.append(ksrc)

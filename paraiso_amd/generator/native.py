@@ -44,6 +44,9 @@ class Setup:
     # block tiles: launch as many workgroups as fit on the device at once and let each loop
     # over tiles, loading the next tile's fields while it computes the current one
     persist: bool = False
+    # per OM kernel, a launch form overriding the above: ((kernel name, {"tile": ...,
+    # "threads": ..., "persist": ..., "march": ..., "mat_gain": ...}), ...)
+    kernel_forms: Tuple[Tuple[str, dict], ...] = ()
 
     def __post_init__(self):
         self.local_size = tuple(int(x) for x in self.local_size)
@@ -61,6 +64,15 @@ class Setup:
 
     def with_(self, **kw) -> "Setup":
         return replace(self, **kw)
+
+    def form_of(self, kernel: str) -> dict:
+        """The launch form of one OM kernel: the machine's, with the kernel's overrides."""
+        f = {"tile": self.tile, "threads": self.threads, "persist": self.persist,
+             "march": self.march, "mat_gain": self.mat_gain}
+        for name, over in self.kernel_forms:
+            if name == kernel:
+                f.update(over)
+        return f
 
 
 def default_setup(local_size) -> Setup:

@@ -54,7 +54,8 @@ RUNTIME_SYMBOLS = ["pr_create", "pr_destroy", "pr_set_stream", "pr_sync", "pr_la
                    "pr_static_read", "pr_static_bytes", "pr_size", "pr_memory_size", "pr_cells",
                    "pr_state_save", "pr_state_load",
                    "pr_snapshot_alloc", "pr_snapshot_save", "pr_snapshot_load",
-                   "pr_adjoint_forward", "pr_adjoint_backward",
+                   "pr_adjoint_forward", "pr_adjoint_backward", "pr_adjoint_forward_seq",
+                   "pr_adjoint_backward_seq",
                    "pr_batch_create", "pr_batch_destroy", "pr_batch_grid", "pr_batch_run",
                    "pr_batch_run_graph", "pr_batch_run_timed", "pr_batch_sync",
                    "pr_batch_last_error", "pr_batch_cells"]
@@ -94,6 +95,9 @@ def load_runtime(path: Optional[str] = None) -> ctypes.CDLL:
     lib.pr_snapshot_alloc.argtypes = [vp, i, ctypes.POINTER(i), i]
     lib.pr_adjoint_forward.argtypes = [vp, i, i, i]
     lib.pr_adjoint_backward.argtypes = [vp, i, i, i, i]
+    ip = ctypes.POINTER(i)
+    lib.pr_adjoint_forward_seq.argtypes = [vp, ip, i, i, i]
+    lib.pr_adjoint_backward_seq.argtypes = [vp, ip, i, i, ip, i, i, i]
     lib.pr_batch_create.argtypes = [vp, i, ctypes.POINTER(i), ctypes.POINTER(vp)]
     lib.pr_batch_destroy.argtypes = [vp]
     lib.pr_batch_grid.argtypes = [vp, i]
@@ -193,6 +197,13 @@ class Machine:
 
     def set_stream(self, stream_handle: int):
         self._check(self.rt.pr_set_stream(self.h, ctypes.c_void_p(stream_handle)), "set_stream")
+        self._stream = stream_handle
+
+    def use_stream(self, stream_handle: int):
+        """Issue the machine's work on ``stream_handle`` from now on (set_stream, which waits
+        for the current stream and drops cached graphs, only when it differs)."""
+        if getattr(self, "_stream", None) != stream_handle:
+            self.set_stream(stream_handle)
 
     # ---------------------------------------------------------------- statics
     def _np_shape(self, s):
@@ -256,14 +267,24 @@ class Machine:
         """Restore the statics from a file written by `save` of the same machine and size."""
         self._check(self.rt.pr_state_load(self.h, os.fsencode(path)), "state load")
 
+    def _kernels(self, ks):
+        ks = [ks] if isinstance(ks, (str, int)) else list(ks)
+        return (ctypes.c_int * len(ks))(*[self.kernel_index(k) for k in ks]), len(ks)
+
     def adjoint_forward(self, fwd, nsteps: int, every: int):
-        self._check(self.rt.pr_adjoint_forward(self.h, self.kernel_index(fwd), nsteps, every),
+        """``nsteps`` steps of ``fwd`` (a kernel or a sequence of kernels), a checkpoint
+        every ``every`` steps (pr_adjoint_forward_seq)."""
+        f, nf = self._kernels(fwd)
+        self._check(self.rt.pr_adjoint_forward_seq(self.h, f, nf, nsteps, every),
                     "adjoint_forward")
 
-    def adjoint_backward(self, fwd, adj, nsteps: int, every: int):
-        self._check(self.rt.pr_adjoint_backward(self.h, self.kernel_index(fwd),
-                                                self.kernel_index(adj), nsteps, every),
-                    "adjoint_backward")
+    def adjoint_backward(self, fwd, adj, nsteps: int, every: int, tape_after: int = 0):
+        """The checkpointed adjoint (pr_adjoint_backward_seq): ``fwd`` and ``adj`` are a
+        kernel or a sequence; the adjoint sees the state after ``fwd[:tape_after]``."""
+        f, nf = self._kernels(fwd)
+        a, na = self._kernels(adj)
+        self._check(self.rt.pr_adjoint_backward_seq(self.h, f, nf, tape_after, a, na, nsteps,
+                                                    every), "adjoint_backward")
 
 
 class _GridView(Machine):

@@ -19,7 +19,13 @@ Parameters that the reference fixes in source and the benchmarks vary:
   from an extra scalar static ``dt``, for differentiable runs),
 * ``legacy``: the older API of ``examples-old/Hydro-exampled/HydroMain.hs``
   (typed ``loadIndex``/``loadSize``, ``abs x = max x (negate x)``, no Manifest
-  annotation on the HLLC selector).
+  annotation on the HLLC selector),
+* ``stages`` (fixed dt only): two more kernels that split ``proceed`` at its
+  intermediate state (``cell2`` of ``buildProceed``, HydroMain.hs:156):
+  ``proceed_stage1`` stores the half-step state into four more Array statics
+  ``stage_*``, and ``proceed_stage2`` reads them and finishes the step.  Run in
+  that order they store what ``proceed`` stores, with the same arithmetic; the
+  adjoint is built per stage from them.
 """
 from __future__ import annotations
 
@@ -224,7 +230,7 @@ def bind_conserved(ctx, d, m, e):
 # --------------------------------------------------------------------- kernels
 class HydroProgram:
     def __init__(self, real=CType.Double, gamma=Fraction(5, 3), legacy=False,
-                 dt_mode="cfl", name="Hydro"):
+                 dt_mode="cfl", name="Hydro", stages=False):
         self.ctx = _Ctx(real, gamma, legacy)
         self.real = real
         self.legacy = legacy
@@ -242,6 +248,14 @@ class HydroProgram:
             + [B.static("pressure", Realm.Array, R)])
         if dt_mode == "fixed":
             self.statics.append(B.static("dt", Realm.Scalar, R))
+        if stages and dt_mode != "fixed":
+            raise ValueError("the two-stage kernels need a fixed dt")
+        self.stages = stages
+        if stages:
+            self.statics += ([B.static("stage_density", Realm.Array, R)]
+                             + [B.static("stage_velocity%d" % i, Realm.Array, R)
+                                for i in range(NDIM)]
+                             + [B.static("stage_pressure", Realm.Array, R)])
         self._by_name = {s.name: s for s in self.statics}
 
     def s(self, name):
@@ -294,17 +308,61 @@ class HydroProgram:
         icoord = [B.bind(self._icoord(ax)) for ax in range(NDIM)]
         isize = [B.bind(self._isize(ax)) for ax in range(NDIM)]
         coord = [B.bind(dR[i] * icoord[i]) for i in range(NDIM)]
-        vplus = [B.imm(6), B.imm(0)]
-        vminus = [B.imm(0), B.imm(0)]
+        # the immediates carry the content type, which Haskell infers from the record's type
+        # and which the selects below would otherwise learn only from the slots a kernel uses
+        R = self.real
+        vplus = [B.imm(6, R), B.imm(0, R)]
+        vminus = [B.imm(0, R), B.imm(0, R)]
         region = B.bind(self._region(coord, extent))
         cell0 = bind_primitive(ctx,
-                               ctx.k_gamma() * ctx.k_gamma() * B.select(region, 1, 10),
+                               ctx.k_gamma() * ctx.k_gamma()
+                               * B.select(region, B.imm(1, R), B.imm(10, R)),
                                [B.select(region, vplus[i], vminus[i]) for i in range(NDIM)],
-                               B.imm(0.6))
+                               B.imm(0.6, R))
         out_of = B.bind(
             B.foldl1(lambda a, b: a | b, [icoord[i].lt(0) for i in range(NDIM)])
             | B.foldl1(lambda a, b: a | b, [icoord[i].ge(isize[i]) for i in range(NDIM)]))
         return lift_a(ctx, lambda a, b: B.manifest(B.select(out_of, a, b)), cell0, cell)
+
+    def _step_inputs(self):
+        """Loads, boundary condition and dt shared by ``proceed`` and its two stages."""
+        ctx = self.ctx
+        dens = B.bind(self.load_real("density"))
+        velo = [B.bind(self.load_real("velocity%d" % i)) for i in range(NDIM)]
+        pres = B.bind(self.load_real("pressure"))
+        timeG = B.bind(self.load_real("time"))
+        dRG = [B.bind(self.load_real("dR%d" % i)) for i in range(NDIM)]
+        dR = [B.bind(B.broadcast(x)) for x in dRG]
+        cell = self.boundary_condition(bind_primitive(ctx, dens, velo, pres))
+        dtG = B.bind(self.load_real("dt"))
+        dt = B.bind(B.broadcast(dtG))
+        return cell, dR, dt, dtG, timeG
+
+    def build_stage1(self):
+        """The first half of ``buildProceed`` (HydroMain.hs:156): the order-1 half step
+        from the state, stored into ``stage_*``."""
+        cell, dR, dt, _, _ = self._step_inputs()
+        cell2 = self.proceed_single(1, dt / 2, dR, cell, cell)
+        B.store(self.s("stage_density"), cell2.density())
+        for i in range(NDIM):
+            B.store(self.s("stage_velocity%d" % i), cell2.velocity(i))
+        B.store(self.s("stage_pressure"), cell2.pressure())
+
+    def build_stage2(self):
+        """The second half of ``buildProceed`` (HydroMain.hs:157-164): the order-2 step
+        from the stored half-step state, applied to the state."""
+        ctx = self.ctx
+        cell, dR, dt, dtG, timeG = self._step_inputs()
+        cell2 = bind_primitive(ctx, B.bind(self.load_real("stage_density")),
+                               [B.bind(self.load_real("stage_velocity%d" % i))
+                                for i in range(NDIM)],
+                               B.bind(self.load_real("stage_pressure")))
+        cell3 = self.proceed_single(2, dt, dR, cell2, cell)
+        B.store(self.s("time"), timeG + dtG)
+        B.store(self.s("density"), cell3.density())
+        for i in range(NDIM):
+            B.store(self.s("velocity%d" % i), cell3.velocity(i))
+        B.store(self.s("pressure"), cell3.pressure())
 
     def build_proceed(self):
         """``buildProceed`` (HydroMain.hs:134-164)."""
@@ -445,13 +503,16 @@ class HydroProgram:
         return lift_a(ctx, selector, left, lesta, rista, right).map_bind()
 
     def make(self):
-        return B.make_om(self.name, [], self.statics,
-                         [("init", self.build_init), ("proceed", self.build_proceed)],
-                         ndim=NDIM)
+        kernels = [("init", self.build_init), ("proceed", self.build_proceed)]
+        if self.stages:
+            kernels += [("proceed_stage1", self.build_stage1),
+                        ("proceed_stage2", self.build_stage2)]
+        return B.make_om(self.name, [], self.statics, kernels, ndim=NDIM)
 
 
-def hydro_om(real=CType.Double, gamma=Fraction(5, 3), dt_mode="cfl", name="Hydro"):
-    return HydroProgram(real, gamma, False, dt_mode, name).make()
+def hydro_om(real=CType.Double, gamma=Fraction(5, 3), dt_mode="cfl", name="Hydro",
+             stages=False):
+    return HydroProgram(real, gamma, False, dt_mode, name, stages).make()
 
 
 def hydro_om_legacy():

@@ -69,6 +69,25 @@ def _hydro_adj(real):
     return mk
 
 
+STAGE_FIELDS = ["stage_" + f for f in HYDRO_FIELDS]
+
+
+def _hydro_adj_staged(real):
+    """The fixed-dt step split into its two stages (``proceed_stage1`` stores the half-step
+    state, ``proceed_stage2`` finishes the step from it), with one adjoint kernel per stage:
+    the checkpoint driver replays both stages and runs ``proceed_stage2_adjoint`` then
+    ``proceed_stage1_adjoint``, so no adjoint launch recomputes stage 1 under the halo that
+    stage 2 needs."""
+    def mk():
+        from .adjoint import add_adjoint
+        from .schemes.hydro import hydro_om
+        om = hydro_om(real, Fraction(7, 5), "fixed", stages=True)
+        om = add_adjoint(om, "proceed_stage2", HYDRO_FIELDS + STAGE_FIELDS,
+                         zero_in=STAGE_FIELDS)
+        return add_adjoint(om, "proceed_stage1", HYDRO_FIELDS + STAGE_FIELDS)
+    return mk
+
+
 def _wave_adj(ndim, real):
     """LinearWave with its adjoint, and both fused two steps per launch (``proceed_x2``,
     ``proceed_adjoint_x2``; more steps per launch need deeper halos than a streaming tile's
@@ -112,6 +131,11 @@ VARIANTS = {
                               persist=True), True),
     "hydro_adj_f64": (_hydro_adj(CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
                                                       tile=(32, 8)), True),
+    # the adjoint per stage (see _hydro_adj_staged), FP32 fast and FP64 correctly rounded
+    "hydro_adj2_f32_fast": (_hydro_adj_staged(CType.Float),
+                            Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 8)), False),
+    "hydro_adj2_f64": (_hydro_adj_staged(CType.Double),
+                       Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 8)), True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
     "wave3d_f64": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP, boundary=CYC3), True),
     # streaming along the last axis (rings of planes in LDS), checked bit for bit

@@ -98,3 +98,99 @@ def test_fp32_adjoint_close_to_fp64(require_gpu, variant):
     g = fields[0].grad.double().cpu().numpy()
     err = np.abs(g - g_ref[0]).sum() / np.abs(g_ref[0]).sum()
     assert err < 1e-3, err
+
+
+# ---------------------------------------------------------------- the adjoint per stage
+STAGED = dict(fwd=["proceed_stage1", "proceed_stage2"],
+              adj=["proceed_stage2_adjoint", "proceed_stage1_adjoint"], tape_after=1)
+
+
+def test_stages_compose_to_proceed(require_gpu):
+    """proceed_stage1 then proceed_stage2 store what proceed stores, bit for bit (correctly
+    rounded FP64: the same operations on the same values)."""
+    nx, ny, dt, steps = 96, 64, 1e-3, 7
+    print("stages vs proceed %dx%d" % (nx, ny), flush=True)
+    state = generic_state(nx, ny, 3)
+    a, b = machine("hydro_adj2_f64", nx, ny, dt), machine("hydro_adj2_f64", nx, ny, dt)
+    for m in (a, b):
+        for f, arr in zip(F, state):
+            m.set(f, arr)
+    a.run("proceed", steps)
+    for _ in range(steps):
+        b.run("proceed_stage1")
+        b.run("proceed_stage2")
+    assert a.get("time") == b.get("time")
+    for f in F:
+        assert np.array_equal(a.get(f), b.get(f)), f
+
+
+@pytest.mark.parametrize("nsteps,every", [(10, 4), (6, 6), (7, 1)])
+def test_staged_adjoint_matches_cpu_fp64_reference(require_gpu, nsteps, every):
+    """The adjoint per stage, driven by the tape-rotating checkpoint replay, against the CPU
+    FP64 reference adjoint."""
+    nx, ny, dt = 32, 32, 1e-3
+    print("staged adjoint nsteps=%d every=%d" % (nsteps, every), flush=True)
+    state = generic_state(nx, ny, 2)
+    J_ref, g_ref, _ = cpu_reference_gradient(state, nx, ny, nsteps, dt)
+    m = machine("hydro_adj2_f64", nx, ny, dt)
+    fields = [torch.tensor(a, dtype=torch.float64, device="cuda", requires_grad=True)
+              for a in state]
+    out = rollout(m, F, fields, nsteps, every, **STAGED)
+    _, mask = HT.kinetic_energy_in_square(*[o.detach().cpu() for o in out[:3]])
+    J = objective(out, mask.cuda().to(torch.float64))
+    J.backward()
+    assert abs(float(J) - J_ref) <= 1e-12 * abs(J_ref)
+    for f, g, gr in zip(F, fields, g_ref):
+        err = np.abs(g.grad.cpu().numpy() - gr).max() / np.abs(gr).max()
+        assert err < 1e-10, (f, err)
+
+
+def test_staged_fp32_adjoint_close_to_fp64(require_gpu):
+    nx, ny, dt, nsteps, every = 32, 32, 1e-3, 8, 4
+    state = generic_state(nx, ny, 9)
+    _, g_ref, _ = cpu_reference_gradient(state, nx, ny, nsteps, dt)
+    m = machine("hydro_adj2_f32_fast", nx, ny, dt)
+    fields = [torch.tensor(a, dtype=torch.float32, device="cuda", requires_grad=True)
+              for a in state]
+    out = rollout(m, F, fields, nsteps, every, **STAGED)
+    _, mask = HT.kinetic_energy_in_square(*[o.detach().cpu().double() for o in out[:3]])
+    objective(out, mask.cuda().to(torch.float32)).backward()
+    for f, x, gr in zip(F, fields, g_ref):
+        g = x.grad.double().cpu().numpy()
+        err = np.abs(g - gr).sum() / np.abs(gr).sum()
+        assert err < 1e-3, (f, err)
+
+
+def test_rollouts_of_one_machine_do_not_mix(require_gpu):
+    """A second rollout of the same machine overwrites the first one's checkpoints: the
+    first one's backward raises instead of returning a gradient of the other run."""
+    nx, ny, dt = 24, 24, 1e-3
+    m = machine("hydro_adj_f64", nx, ny, dt)
+    x1 = [torch.tensor(a, device="cuda", requires_grad=True) for a in generic_state(nx, ny, 1)]
+    x2 = [torch.tensor(a, device="cuda", requires_grad=True) for a in generic_state(nx, ny, 2)]
+    o1 = rollout(m, F, x1, 3, 2)
+    o2 = rollout(m, F, x2, 3, 2)
+    o2[0].sum().backward()
+    assert x2[0].grad is not None
+    with pytest.raises(RuntimeError, match="overwritten"):
+        o1[0].sum().backward()
+
+
+def test_rollout_with_strided_grad_and_transposed_input(require_gpu):
+    """J = out[0].sum() gives a stride-0 (expanded) gradient, and a transposed input is
+    not contiguous: both are made contiguous on torch's stream, which the machine shares."""
+    nx, ny, dt, nsteps, every = 24, 24, 1e-3, 4, 2
+    state = generic_state(nx, ny, 4)
+    m = machine("hydro_adj_f64", nx, ny, dt)
+    base = [torch.tensor(a.T.copy(), device="cuda") for a in state]     # (nx, ny) storage
+    x = [b.t().requires_grad_() for b in base]                           # (ny, nx) views
+    out = rollout(m, F, x, nsteps, every)
+    out[0].sum().backward()
+    # the same J through the CPU FP64 reference
+    t = [torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in state]
+    o, _ = HT.run(t, nsteps, (1.0 / nx, 1.0 / ny), 1.4, dt=dt)
+    o[0].sum().backward()
+    for f, xi, ti in zip(F, x, t):
+        err = np.abs(xi.grad.cpu().numpy() - ti.grad.numpy()).max() / max(
+            np.abs(ti.grad.numpy()).max(), 1e-300)
+        assert err < 1e-10, (f, err)
