@@ -54,6 +54,17 @@ CANDIDATES = {
 }
 
 
+# Candidates for machines with several kernels of different weight (an adjoint per stage):
+# each kernel keeps the launch form that ran it fastest (select --per-kernel).
+CANDIDATE_SETS = {
+    "per_kernel_2d": [((32, 8), 256), ((16, 16), 256), ((64, 4), 256), ((64, 8), 256),
+                      ((32, 16), 256), ((32, 16), 512), ((64, 8), 512), ((64, 16), 512),
+                      ((32, 32), 512), ((64, 16), 1024), ((32, 32), 1024), ((64, 32), 1024),
+                      ((128, 8), 1024), ((32, 8), 256, 24), ((32, 8), 256, 96),
+                      ((64, 8), 256, 96)],
+}
+
+
 def tuned() -> Dict:
     if os.path.exists(TUNED):
         with open(TUNED) as f:
@@ -126,12 +137,14 @@ def _setup_inputs(variant: str, m, size):
 
 
 def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=None,
-            kernel: str = None, batch: int = 0, libs_of: str = None) -> List[Dict]:
+            kernel: str = None, batch: int = 0, libs_of: str = None,
+            kernels: List[str] = None) -> List[Dict]:
     """Device time per step of every compiled candidate; the step is ``proceed_pipelined``
     (after ``proceed_prime``) where the program has it, else ``kernel`` or ``proceed``.
     ``batch`` > 0 times a batch of that many grids of log-uniform sides on [64, 1024]
     (seed 2024) instead of one grid of ``size``; ``libs_of`` names the variant whose
-    compiled candidates are timed (default: ``variant``)."""
+    compiled candidates are timed (default: ``variant``).  ``kernels`` also times each of
+    those OM kernels on its own (``per_kernel_ms``), for a launch form per kernel."""
     from . import inputs, variants
     from .generator import GEN_DIR
     from .machine import Batch, Machine
@@ -162,12 +175,42 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
         m.run(step, warmup)
         m.sync()
         ms = m.run_timed(step, steps)
-        results.append({"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
-                        "kernel": step,
-                        "ms_per_step": sum(ms) / steps,
-                        "launch_ms": [x / steps for x in ms]})
+        rec = {"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
+               "kernel": step, "ms_per_step": sum(ms) / steps,
+               "launch_ms": [x / steps for x in ms]}
+        if kernels:
+            rec["per_kernel_ms"] = {}
+            for k in kernels:
+                m.run(k, 2)
+                rec["per_kernel_ms"][k] = sum(m.run_timed(k, steps)) / steps
+        results.append(rec)
         m.close()
     return results
+
+
+def select_per_kernel(path: str) -> Dict:
+    """Record, per OM kernel, the candidate launch form that ran that kernel fastest
+    (``kernel_forms`` of the variant in tuned.json); the machine is then built with one
+    launch form per kernel."""
+    with open(path) as f:
+        data = json.load(f)
+    t = tuned()
+    for variant, rec in data.items():
+        res = [r for r in rec["results"] if "per_kernel_ms" in r]
+        forms, table = {}, {}
+        for k in res[0]["per_kernel_ms"]:
+            best = min(res, key=lambda r: r["per_kernel_ms"][k])
+            forms[k] = {"tile": best["tile"], "threads": best["threads"],
+                        "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
+                        "ms": round(best["per_kernel_ms"][k], 5)}
+            table[k] = {"%s_n%d_g%d_m%d" % ("x".join(map(str, r["tile"])), r["threads"],
+                                            r.get("mat_gain", 48), r.get("march", 0)):
+                        round(r["per_kernel_ms"][k], 5) for r in res}
+        t[variant] = {"kernel_forms": forms, "size": rec["size"], "steps": rec["steps"],
+                      "candidates_per_kernel": table}
+    with open(TUNED, "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+    return t
 
 
 def select(path: str) -> Dict:
@@ -194,6 +237,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     p = sub.add_parser("prepare")
     p.add_argument("--variant", required=True)
+    p.add_argument("--set", default=None, help="a named candidate set (CANDIDATE_SETS)")
     q = sub.add_parser("measure")
     q.add_argument("--variant", action="append", required=True)
     q.add_argument("--size", type=int, nargs="+", action="append", required=True)
@@ -201,12 +245,17 @@ def main(argv=None):
     q.add_argument("--kernel", default=None, help="kernel to time (default: the step)")
     q.add_argument("--batch", type=int, default=0, help="time a batch of this many grids")
     q.add_argument("--libs-of", default=None, help="variant whose candidates are timed")
+    q.add_argument("--kernels", nargs="*", default=None,
+                   help="also time each of these kernels on its own")
+    q.add_argument("--set", default=None, help="a named candidate set (CANDIDATE_SETS)")
     q.add_argument("--out", required=True)
     s = sub.add_parser("select")
     s.add_argument("path")
+    s.add_argument("--per-kernel", action="store_true",
+                   help="record the fastest candidate of each kernel (kernel_forms)")
     a = ap.parse_args(argv)
     if a.cmd == "prepare":
-        for c, so in prepare(a.variant):
+        for c, so in prepare(a.variant, CANDIDATE_SETS.get(a.set)):
             print(c, so)
     elif a.cmd == "measure":
         out = {}
@@ -214,9 +263,13 @@ def main(argv=None):
             out[variant] = {"size": size if not a.batch else ["batch", a.batch],
                             "steps": a.steps,
                             "results": measure(variant, size, a.steps, kernel=a.kernel,
-                                               batch=a.batch, libs_of=a.libs_of)}
+                                               batch=a.batch, libs_of=a.libs_of,
+                                               kernels=a.kernels,
+                                               candidates=CANDIDATE_SETS.get(a.set))}
             with open(a.out, "w") as f:
                 json.dump(out, f, indent=1)
+    elif a.per_kernel:
+        print(json.dumps(select_per_kernel(a.path), indent=1))
     else:
         print(json.dumps(select(a.path), indent=1))
 

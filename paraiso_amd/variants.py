@@ -163,10 +163,18 @@ def setup_of(name: str):
     from .autotune import tuned
     mk, setup, precise = VARIANTS[name]
     rec = tuned().get(name)
-    if rec is not None:
+    if rec is not None and "tile" in rec:
         setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)),
                             mat_gain=int(rec.get("mat_gain", 48)), march=int(rec.get("march", 0)),
                             persist=bool(rec.get("persist", setup.persist)))
+    if rec is not None and "kernel_forms" in rec:
+        forms = []
+        for k, f in sorted(rec["kernel_forms"].items()):
+            forms.append((k, {"tile": tuple(f["tile"]), "threads": int(f["threads"]),
+                              "mat_gain": int(f.get("mat_gain", 48)),
+                              "march": int(f.get("march", 0)),
+                              "persist": bool(f.get("persist", False))}))
+        setup = setup.with_(kernel_forms=tuple(forms))
     return setup
 
 
