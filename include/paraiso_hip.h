@@ -206,6 +206,29 @@ int pr_adjoint_forward_seq(pr_machine* m, const int* fwd, int n_fwd, int nsteps,
 int pr_adjoint_backward_seq(pr_machine* m, const int* fwd, int n_fwd, int tape_after,
                             const int* adj, int n_adj, int nsteps, int every);
 
+/* ---- running without allocating (no counterpart in the reference: for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Runs `count` calls of kernel `k` of the machine `desc` on a
+ * grid of `local_size` cells, on storage the caller owns, and allocates nothing:
+ *   buffers  2 * n_static device pointers; buffers[2s] and buffers[2s + 1] are the two
+ *            buffers of array static s (each prod(memory extent) elements of its type;
+ *            the second may be NULL unless kernel k stores s); scalar entries are ignored;
+ *   parity   n_static ints: which of the two holds static s on entry; updated on return
+ *            (a store writes the other one, as pr_run's ping-pong does);
+ *   scalars  device block of 8 bytes per static, scalar static s at byte 8 s (the layout
+ *            pr_static_devptr exposes): read before the first call, written after the last;
+ *   stream   hipStream_t the work is issued on (NULL: the default stream);
+ *   err      optional buffer for a message on failure.
+ * The scalar slots, reduction partials and scratch arrays the kernels need live in one
+ * fixed device array of PR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * a kernel and grid that need more are rejected with PR_E_INVALID before any work is
+ * issued.  Calls on one GPU share that array: calls that overlap on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+#define PR_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pr_run_noalloc(const pr_machine_desc* desc, const int* local_size, int k, int count,
+                   void* const* buffers, int* parity, void* scalars, void* stream,
+                   char* err, size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

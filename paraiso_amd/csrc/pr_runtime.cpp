@@ -14,11 +14,13 @@
 //   * reduction partials live in one buffer sized from the tile counts.
 #include <hip/hip_runtime.h>
 
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <array>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -1030,3 +1032,186 @@ long long pr_batch_cells(const pr_batch* b) { return b ? b->cells : 0; }
 
 }  // extern "C"
 
+// ---------------------------------------------------------------- running without allocating
+// pr_run_noalloc runs a kernel on buffers the caller owns.  What pr_create would allocate
+// besides the statics -- the scalar block (scalar statics, scalar slots the finish kernels
+// write, per-launch counters), the reduction partials and the scratch arrays of cut
+// launches -- lives in `scratch`, a fixed array every GPU holds for the life of the process.
+// Host code must never pass `scratch` itself to a kernel or a copy: its name is a host-side
+// stand-in, not the device address, which hipGetSymbolAddress gives per device.
+
+static __device__ __align__(256) char scratch[PR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int PR_NOALLOC_MAX_DEVICES = 64;
+constexpr int PR_NOALLOC_MAX_STATICS = 256;
+constexpr int PR_NOALLOC_MAX_LAUNCHES = 32;
+std::mutex noalloc_mu;
+char* noalloc_scratch[PR_NOALLOC_MAX_DEVICES];   // device address of `scratch`, per device
+
+int noalloc_fail(char* err, size_t len, int code, const char* fmt, ...) {
+  if (err && len) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(err, len, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" int pr_run_noalloc(const pr_machine_desc* d, const int* local_size, int k, int count,
+                              void* const* buffers, int* parity, void* scalars, void* stream,
+                              char* err, size_t err_len) {
+  if (err && err_len) err[0] = 0;
+  if (!d || !local_size || !buffers || !parity || count < 0)
+    return noalloc_fail(err, err_len, PR_E_INVALID, "null argument or negative count");
+  if (d->abi != PR_ABI_VERSION) return noalloc_fail(err, err_len, PR_E_ABI, "descriptor ABI");
+  if (k < 0 || k >= d->n_kernel)
+    return noalloc_fail(err, err_len, PR_E_INVALID, "kernel index %d out of range", k);
+  const int ns = d->n_static;
+  const pr_kernel_desc& kd = d->kernels[k];
+  if (ns > PR_NOALLOC_MAX_STATICS || kd.n_launch > PR_NOALLOC_MAX_LAUNCHES)
+    return noalloc_fail(err, err_len, PR_E_UNSUPPORTED, "machine too large for pr_run_noalloc");
+  // grid extents, as pr_create
+  int n[3] = {1, 1, 1};
+  long long cells = 1;
+  for (int a = 0; a < d->ndim; ++a) {
+    if (local_size[a] < 1 || local_size[a] < d->halo)
+      return noalloc_fail(err, err_len, PR_E_INVALID,
+                          "grid extent %d on axis %d is below the machine's halo %d",
+                          local_size[a], a, d->halo);
+    n[a] = local_size[a] + d->lower_margin[a] + d->upper_margin[a];
+    cells *= n[a];
+  }
+  if (cells >= (1LL << 31))
+    return noalloc_fail(err, err_len, PR_E_INVALID, "grid of %lld cells exceeds 2^31 - 1", cells);
+  // scratch layout: scalar block, this kernel's partials, scratch arrays
+  size_t n_counter = 0, counter0 = 0;
+  for (int kk = 0; kk < d->n_kernel; ++kk) {
+    if (kk == k) counter0 = n_counter;
+    n_counter += (size_t)d->kernels[kk].n_launch;
+  }
+  const size_t scal_bytes = round256(8 * ((size_t)ns + d->n_slot + n_counter));
+  size_t need = scal_bytes;
+  size_t part_off[PR_NOALLOC_MAX_LAUNCHES];
+  unsigned ntiles[PR_NOALLOC_MAX_LAUNCHES];
+  int tiles[PR_NOALLOC_MAX_LAUNCHES][3];
+  for (int l = 0; l < kd.n_launch; ++l) {
+    const pr_launch_desc& ld = kd.launches[l];
+    unsigned nt = 1;
+    for (int a = 0; a < 3; ++a) {
+      const int ta = ld.tile[a] > 0 ? ld.tile[a] : 1;
+      tiles[l][a] = ld.tile[0] > 0 ? (n[a] + ta - 1) / ta : 1;
+      nt *= (unsigned)tiles[l][a];
+    }
+    ntiles[l] = nt;
+    part_off[l] = need;
+    need += round256((size_t)ld.n_reduce * nt * 8);
+  }
+  size_t array_off[PR_NOALLOC_MAX_STATICS];
+  for (int s = 0; s < ns; ++s) {
+    const pr_static_desc& sd = d->statics[s];
+    array_off[s] = 0;
+    if (sd.realm == PR_ARRAY && (sd.flags & PR_STATIC_SCRATCH)) {
+      array_off[s] = need;
+      need += round256((size_t)cells * ctype_size(sd.ctype));
+    }
+  }
+  if (need > (size_t)PR_NOALLOC_SCRATCH_BYTES)
+    return noalloc_fail(err, err_len, PR_E_INVALID,
+                        "kernel %s on this grid needs %zu bytes of scratch; pr_run_noalloc "
+                        "holds %zu (PR_NOALLOC_SCRATCH_BYTES): use pr_create and pr_run",
+                        kd.name, need, (size_t)PR_NOALLOC_SCRATCH_BYTES);
+  // the caller's buffers: one per array static, a second for those this kernel stores
+  bool stored[PR_NOALLOC_MAX_STATICS] = {};
+  for (int i = 0; i < kd.n_stored; ++i) stored[kd.stored[i]] = true;
+  bool any_scalar = false;
+  for (int s = 0; s < ns; ++s) {
+    const pr_static_desc& sd = d->statics[s];
+    if (sd.realm != PR_ARRAY) {
+      any_scalar = true;
+      continue;
+    }
+    if (sd.flags & PR_STATIC_SCRATCH) continue;
+    if (parity[s] != 0 && parity[s] != 1)
+      return noalloc_fail(err, err_len, PR_E_INVALID, "parity of %s is not 0 or 1", sd.name);
+    if (!buffers[2 * s + parity[s]])
+      return noalloc_fail(err, err_len, PR_E_INVALID, "no buffer holds %s", sd.name);
+    if (stored[s] && !buffers[2 * s + 1 - parity[s]])
+      return noalloc_fail(err, err_len, PR_E_INVALID,
+                          "kernel %s stores %s: it needs a second buffer", kd.name, sd.name);
+  }
+  if (any_scalar && !scalars)
+    return noalloc_fail(err, err_len, PR_E_INVALID, "the machine has scalar statics: pass them");
+  // this device's copy of `scratch`
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return noalloc_fail(err, err_len, PR_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= PR_NOALLOC_MAX_DEVICES)
+    return noalloc_fail(err, err_len, PR_E_UNSUPPORTED, "device %d", dev);
+  char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_scratch[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess)
+        return noalloc_fail(err, err_len, PR_E_HIP, "hipGetSymbolAddress: %s", hipGetErrorString(e));
+      noalloc_scratch[dev] = static_cast<char*>(p);
+    }
+    base = noalloc_scratch[dev];
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  char* scal = base;
+  if (any_scalar) {
+    e = hipMemcpyAsync(scal, scalars, 8 * (size_t)ns, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess)
+      return noalloc_fail(err, err_len, PR_E_HIP, "scalars in: %s", hipGetErrorString(e));
+  }
+  const void* cur[PR_NOALLOC_MAX_STATICS];
+  void* next[PR_NOALLOC_MAX_STATICS];
+  for (int c = 0; c < count; ++c) {
+    for (int s = 0; s < ns; ++s) {
+      const pr_static_desc& sd = d->statics[s];
+      cur[s] = nullptr;
+      next[s] = nullptr;
+      if (sd.realm != PR_ARRAY) continue;
+      if (sd.flags & PR_STATIC_SCRATCH) {
+        cur[s] = base + array_off[s];        // cut launches write these through cur
+        continue;
+      }
+      cur[s] = buffers[2 * s + parity[s]];
+      if (stored[s]) next[s] = buffers[2 * s + 1 - parity[s]];
+    }
+    for (int l = 0; l < kd.n_launch; ++l) {
+      pr_launch_ctx ctx;
+      ctx.cur = cur;
+      ctx.next = next;
+      ctx.scal = scal;
+      ctx.part = base + part_off[l];
+      ctx.counter = reinterpret_cast<unsigned*>(scal + 8 * ((size_t)ns + d->n_slot + counter0 + l));
+      for (int a = 0; a < 3; ++a) {
+        ctx.n[a] = n[a];
+        ctx.tiles[a] = tiles[l][a];
+      }
+      ctx.ntiles = ntiles[l];
+      ctx.stream = st;
+      if (kd.launches[l].fn(&ctx) != 0)
+        return noalloc_fail(err, err_len, PR_E_HIP, "launch %s[%d]: %s", kd.name, l,
+                            hipGetErrorString(hipGetLastError()));
+    }
+    for (int i = 0; i < kd.n_stored; ++i) parity[kd.stored[i]] ^= 1;
+  }
+  if (any_scalar) {
+    e = hipMemcpyAsync(scalars, scal, 8 * (size_t)ns, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess)
+      return noalloc_fail(err, err_len, PR_E_HIP, "scalars out: %s", hipGetErrorString(e));
+  }
+  return PR_OK;
+}
