@@ -14,11 +14,16 @@ reference's generated C++, OpenMP on the host cores) on a bounded sample.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on neighbouring cores; read when libgomp loads
+os.environ.setdefault("OMP_PROC_BIND", "close")
+# CPUs this process may use, before an OpenMP runtime binds the main thread to one of them
+AFFINITY_CPUS = len(os.sched_getaffinity(0))
 
 HBM_PEAK_GBPS = 8000.0
 FIELDS = ("density", "velocity0", "velocity1", "pressure")
@@ -79,6 +84,37 @@ def time_machine(variant, nx, ny, steps, warmup, seed, dtype, bytes_per_cell,
 
 
 VALU_PEAK = 256 * 4 * 2.4e9 / 4      # wave64 VALU instructions/s, gfx950 without packed FP32
+LANE_VALU_PEAK = VALU_PEAK * 64       # lane instructions/s: 256 CUs x 4 SIMDs x 16 lanes
+PROFILE = "profiles/hydro2d_kh_2048x2048_fp32.md"
+
+
+def useful_ops_per_cell(variant, kernel):
+    """Operations per cell of an OM kernel as written, each value once per cell (the lowered
+    IR, after value numbering and constant folding): floating arithmetic (add, sub, mul,
+    div, sqrt, min, max, abs, neg) and compares, selects and logic.  This is the work a cell
+    update needs without halo recomputation, address arithmetic or data movement."""
+    from paraiso_amd import variants
+    from paraiso_amd.generator.hip.ir import lower_kernel
+    from paraiso_amd.generator.plan import translate
+    from paraiso_amd.om.arith import COMPARISONS, Op
+    from paraiso_amd.om.dyn import Realm
+    mk, _, precise = variants.VARIANTS[variant]
+    plan = translate(variants.setup_of(variant), mk())
+    om = plan.om
+    ki = [k.name for k in om.kernels].index(kernel)
+    lk = lower_kernel(om, ki, (True,) * om.ndim, exact=not precise)
+    fp = {Op.Add, Op.Sub, Op.Mul, Op.Div, Op.Inv, Op.Neg, Op.Sqrt, Op.Abs, Op.Max, Op.Min}
+    logic = set(COMPARISONS) | {Op.Select, Op.And, Op.Or, Op.Not}
+    out = {"fp_arith": 0, "compare_select": 0}
+    for n in lk.nodes:
+        if n.op != "arith" or n.realm is not Realm.Array:
+            continue
+        op = n.payload[0]
+        if op in fp:
+            out["fp_arith"] += 1
+        elif op in logic:
+            out["compare_select"] += 1
+    return out
 
 
 def build_key(variant):
@@ -91,7 +127,7 @@ def build_key(variant):
 def profile_figures(variant, kernel):
     """HBM bytes and VALU instructions per launch of `kernel`, from the rocprofv3 summary
     committed under profiles/ for this workload, if it was taken with the same tile."""
-    path = os.path.join(ROOT, "profiles", "hydro2d_kh_2048x2048_fp32.md")
+    path = os.path.join(ROOT, PROFILE)
     if not os.path.exists(path):
         return None, None
     from paraiso_amd import variants
@@ -100,7 +136,7 @@ def profile_figures(variant, kernel):
         for line in f:
             for key in ("profile_config", "traffic_bytes_per_launch",
                         "valu_wave_instructions_per_launch", "valu_active_quad_cycles_per_launch",
-                        "valu_transcendental_per_launch"):
+                        "valu_transcendental_per_launch", "valu_mix_per_launch"):
                 if line.startswith(key + ":"):
                     rec[key] = json.loads(line.split(":", 1)[1])
     cfg = rec.get("profile_config", {})
@@ -111,7 +147,7 @@ def profile_figures(variant, kernel):
             or cfg.get("build_key") != build_key(variant)):
         return None, None
     valu = {k: rec.get("valu_%s_per_launch" % k, {}).get(kernel)
-            for k in ("wave_instructions", "active_quad_cycles", "transcendental")}
+            for k in ("wave_instructions", "active_quad_cycles", "transcendental", "mix")}
     return (rec.get("traffic_bytes_per_launch", {}).get(kernel),
             valu if valu["wave_instructions"] is not None else None)
 
@@ -196,36 +232,68 @@ def bench_batched(n_grids=256, steps=200, seed=2024):
     b.sync()
     t1 = time.perf_counter()
     cells = b.cells()
+    # a seeded sample of grids against their own single-grid runs of the same steps
+    pick = sorted(int(i) for i in np.random.default_rng(seed).choice(n_grids, 4, replace=False))
+    worst, same = 0.0, True
+    for g in pick:
+        nx, ny = sizes[g]
+        m = variants.machine("hydro_f32_batch", (nx, ny))
+        setup_hydro(m, nx, ny, seed + g, np.float32)
+        m.run("proceed_prime")
+        m.run("proceed_pipelined", steps)
+        for f in FIELDS:
+            a, c = b.grids[g].get(f), m.get(f)
+            same = same and bool(np.array_equal(a, c))
+            worst = max(worst, float(np.abs(a.astype(np.float64) - c).max()))
+        same = same and b.grids[g].get("time") == m.get("time")
+        m.close()
     ms = b.run_timed("proceed_pipelined", 10)
+    b.close()
     return {"workload": "hydro2d-kh-batched-%d-grids-fp32" % n_grids, "grids": n_grids,
             "cells_per_step": cells, "steps": steps,
             "value": cells * steps / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
             "ms_per_step": (t1 - t0) / steps * 1e3, "launch_ms_per_step": [x / 10 for x in ms],
-            "sizes_seed": seed}
+            "sizes_seed": seed,
+            "single_grid_check": {"grids": pick, "sizes": [list(sizes[g]) for g in pick],
+                                  "bit_identical": same, "max_abs_diff": worst}}
 
 
-def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fast"):
-    """Fixed-dt Hydro 512^2 FP32: 200 forward steps vs forward + checkpointed adjoint."""
+STAGES = ["proceed_stage1", "proceed_stage2"]
+STAGE_ADJOINTS = ["proceed_stage2_adjoint", "proceed_stage1_adjoint"]
+
+
+def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj2_f32_fast"):
+    """Fixed-dt Hydro 512^2 FP32: 200 forward steps against forward + checkpointed adjoint.
+
+    The forward with checkpoints runs the fused step ``proceed``; the backward replays each
+    segment in the step's two stages and runs one adjoint kernel per stage (stage 2, then
+    stage 1), each kernel at the launch form tuned for it.  In correctly rounded builds the
+    two stages store what ``proceed`` stores bit for bit; here (FP32 fast) they agree to
+    rounding."""
     import numpy as np
     import torch
     from paraiso_amd import inputs, variants
     m = variants.machine(variant, (n, n))
-    setup_hydro(m, n, n, 7, np.float32)
-    m.set("dt", dt)
     names = [s[0] for s in m.statics]
     snap = [x for x in names if not x.startswith("adj_") and not x.startswith("__")]
     S = (steps + every - 1) // every
     m.snapshot_alloc(S + every, snap)
-    m.run("proceed", 5)
+
+    def fresh():
+        setup_hydro(m, n, n, 7, np.float32)
+        m.set("dt", dt)
+    fresh()
+    for k in ["proceed"] + STAGES + STAGE_ADJOINTS:       # first launches outside the timing
+        m.run(k, 2)
+    m.snapshot_save(0)                                     # the runtime's copy kernel too
+    m.snapshot_load(0)
     m.sync()
-    setup_hydro(m, n, n, 7, np.float32)
-    m.set("dt", dt)
+    fresh()
     t0 = time.perf_counter()
     m.run("proceed", steps)
     m.sync()
     t_fwd = time.perf_counter() - t0
-    setup_hydro(m, n, n, 7, np.float32)
-    m.set("dt", dt)
+    fresh()
     t0 = time.perf_counter()
     m.adjoint_forward("proceed", steps, every)
     m.sync()
@@ -240,11 +308,13 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
              "adj_velocity1": rho * v1 * mask, "adj_pressure": torch.zeros_like(rho)}
     for k, v in seeds.items():
         m.set(k, v.float().cpu().numpy())
+    m.sync()
     t0 = time.perf_counter()
-    m.adjoint_backward("proceed", "proceed_adjoint", steps, every)
+    m.adjoint_backward(STAGES, STAGE_ADJOINTS, steps, every, tape_after=1)
     m.sync()
     t_ab = time.perf_counter() - t0
     g = m.get("adj_density")
+    kernel_us = {k: 1e3 * sum(m.run_timed(k, 20)) / 20 for k in ["proceed"] + STAGES + STAGE_ADJOINTS}
     m.close()
     # the same fixed-dt forward on the plain FP64 single-grid path (correctly rounded FP64) at
     # its fastest launch form
@@ -260,28 +330,15 @@ def bench_adjoint(n=512, steps=200, every=20, dt=1e-4, variant="hydro_adj_f32_fa
     m64.sync()
     t_fwd64 = time.perf_counter() - t0
     m64.close()
-    # and the FP32 fixed-dt step at the headline's launch form: the forward in the ratio is
-    # the adjoint machine's own, and this shows whether another form would be faster
-    m32 = variants.machine("hydro_fixed_f32", (n, n))
-    setup_hydro(m32, n, n, 7, np.float32)
-    m32.set("dt", dt)
-    m32.run("proceed", 5)
-    m32.sync()
-    setup_hydro(m32, n, n, 7, np.float32)
-    m32.set("dt", dt)
-    t0 = time.perf_counter()
-    m32.run("proceed", steps)
-    m32.sync()
-    t_fwd32 = time.perf_counter() - t0
-    m32.close()
     return {"workload": "hydro2d-adjoint-%dx%d-fp32-fixed-dt" % (n, n), "variant": variant,
-            "steps": steps,
-            "checkpoint_every": every, "fwd_ms": t_fwd * 1e3,
-            "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
+            "steps": steps, "checkpoint_every": every,
+            "forward_kernel": "proceed", "replay_kernels": STAGES,
+            "adjoint_kernels": STAGE_ADJOINTS,
+            "fwd_ms": t_fwd * 1e3, "fwd_with_checkpoints_ms": t_af * 1e3, "adjoint_ms": t_ab * 1e3,
             "fwd_adj_over_fwd": (t_af + t_ab) / t_fwd,
             "fp64_fwd_ms": t_fwd64 * 1e3,
             "fwd_adj_over_fp64_fwd": (t_af + t_ab) / t_fwd64,
-            "fp32_headline_form_fwd_ms": t_fwd32 * 1e3,
+            "kernel_us_per_launch": kernel_us,
             "grad_density_l1": float(np.abs(g).sum())}
 
 
@@ -314,12 +371,15 @@ def _bench_life(n, gens, seed, per_launch):
     pop, gen = int(m.get("population")), int(m.get("generation"))
     ms = m.run_timed(kern, 25)
     cells = n * n
+    launch_s = sum(ms) / 25 * 1e-3
     return {"workload": "life-%dx%d-int32-%d-generations" % (n, n, gens), "generations": gens,
             "value": cells * gens / (t1 - t0) / 1e9, "unit": "Gcell-updates/s",
             "ms_per_step": (t1 - t0) / gens * 1e3,
             "launch_ms_per_launch": [x / 25 for x in ms], "generations_per_launch": per_launch,
             "algorithmic_bytes_per_cell": 8,
-            "achieved_GBps": cells * 8 * gens / (t1 - t0) / 1e9,
+            # a launch of k generations reads and writes each cell once: 8 B per cell moved
+            "hbm_bytes_per_launch": cells * 8,
+            "achieved_GBps": cells * 8 / launch_s / 1e9,
             "population": pop, "generation": gen}
 
 
@@ -413,45 +473,137 @@ def bench_wave(n=512, steps=300, every=20):
     return out
 
 
-def _cpu_model():
+def _cpu_info():
+    """Host CPU as the CPU baseline ran on it: model name, logical CPUs, the CPUs this process
+    may use, and the OpenMP threads and binding of the baseline."""
+    model = "unknown"
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    affinity = AFFINITY_CPUS
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_threads": int(os.environ.get("OMP_NUM_THREADS", affinity)),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "")}
 
 
-def cpu_baseline(nx, ny, runs=3, budget_s=5.0, max_steps=40):
-    """Oracle CPU machine (PlanTrans restatement, OpenMP on every host thread the process
-    may use), bounded sample: `runs` timed runs of about `budget_s` each, median."""
+def _median_rate(run, work, budget_s, max_steps, runs=3):
+    """Median over `runs` of work done per second, each run stepping until `budget_s` or
+    `max_steps`; `run(k)` does k steps, `work` is the work per step."""
+    import numpy as np
+    rates, steps_total, secs = [], 0, 0.0
+    for _ in range(runs):
+        run(1)                                   # first touch outside the timed region
+        steps, t0 = 0, time.perf_counter()
+        while steps < max_steps and time.perf_counter() - t0 < budget_s:
+            run(1)
+            steps += 1
+        dt = time.perf_counter() - t0
+        rates.append(work * steps / dt)
+        steps_total += steps
+        secs += dt
+    return float(np.median(rates)), [round(x, 6) for x in rates], steps_total, secs
+
+
+def cpu_baseline(nx, ny, budget_s=4.0):
+    """The reference's CPU path on the host cores: the oracle's restatement of the reference's
+    generated CPlusPlus class (std::vector storage, SASA loops under OpenMP, serial reduces,
+    deep-copy stores), built g++ -O3 -fopenmp -ffp-contract=off, OMP_PROC_BIND=close, every
+    OpenMP thread the process has.  Bounded samples (median of three runs of about
+    `budget_s` each) of every workload; the headline's figure is the top-level value."""
     import numpy as np
     from fractions import Fraction
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_machine
+    from oracle.prebuild import BASELINE_OPT, batched_sample
+    from paraiso_amd import inputs
     from paraiso_amd.om.dyn import CType
     from paraiso_amd.schemes.hydro import hydro_om
-    m = oracle_machine(lambda: hydro_om(CType.Float, Fraction(7, 5)), (nx, ny))
-    rates, total_steps, total_s = [], 0, 0.0
-    for r in range(runs):
-        setup_hydro(m, nx, ny, 1, np.float32)
-        m.run("proceed", 1)                        # first touch outside the timed region
-        steps, t0 = 0, time.perf_counter()
-        while steps < max_steps and time.perf_counter() - t0 < budget_s:
-            m.run("proceed", 1)
-            steps += 1
-        dt = time.perf_counter() - t0
-        rates.append(nx * ny * steps / dt / 1e9)
-        total_steps += steps
-        total_s += dt
-    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": float(np.median(rates)), "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port", "runs": [round(x, 5) for x in rates], "cpu_model": _cpu_model(),
-            "sample": "median of %d runs, %d steps of Hydro %dx%d in %.1f s, generated by the "
-                      "oracle's restatement of the reference's CPlusPlus emitter (OpenMP, "
-                      "g++ -O2 -ffp-contract=off)" % (runs, total_steps, nx, ny, total_s)}
+    from paraiso_amd.schemes.life import life_om
+    from paraiso_amd.schemes.linear_wave import linear_wave_om
+    G = Fraction(7, 5)
+    info = _cpu_info()
+    flags = "g++ %s -fopenmp -ffp-contract=off" % BASELINE_OPT
+    # Hydro KH 2048^2 FP32 (the headline)
+    m = oracle_machine(lambda: hydro_om(CType.Float, G), (nx, ny), opt=BASELINE_OPT)
+    setup_hydro(m, nx, ny, 1, np.float32)
+    hyd, runs, st, secs = _median_rate(lambda k: m.run("proceed", k), nx * ny / 1e9, budget_s, 40)
+    m.close()
+    out = {"value": hyd, "unit": "Gcell-updates/s", "cores": info["omp_threads"], "kind": "port",
+           "runs": runs, "sample": "median of 3 runs, %d steps of Hydro %dx%d FP32 in %.1f s "
+           "(%s)" % (st, nx, ny, secs, flags)}
+    out.update(info)
+    others = {}
+    # Life 1024^2
+    n = 1024
+    m = oracle_machine(life_om, (n, n), opt=BASELINE_OPT)
+    m.run("init")
+    m.set("cell", inputs.life_init((n, n), 1))
+    v, runs, st, secs = _median_rate(lambda k: m.run("proceed", k), n * n / 1e9, budget_s, 1000)
+    others["life"] = {"value": v, "unit": "Gcell-updates/s", "runs": runs,
+                      "sample": "%d generations of Life %dx%d in %.1f s" % (st, n, n, secs)}
+    m.close()
+    # LinearWave 512^3 FP32 (a stated subset of the 300 steps)
+    n3 = (512, 512, 512)
+    m = oracle_machine(lambda: linear_wave_om(3, CType.Float), n3, opt=BASELINE_OPT)
+    m.set("c", 1.0)
+    m.set("dx", 1.0)
+    m.set("dt", 0.5)
+    m.set("fieldF", inputs.gaussian_pulse(n3, 3, dtype=np.float32))
+    m.set("fieldG", 0.0)
+    v, runs, st, secs = _median_rate(lambda k: m.run("proceed", k), 512 ** 3 / 1e9, budget_s, 20)
+    others["wave3d"] = {"value": v, "unit": "Gcell-updates/s", "runs": runs,
+                        "sample": "%d steps of LinearWave 512^3 FP32 in %.1f s" % (st, secs)}
+    m.close()
+    # batched Hydro: a seeded sample of the 256 grid sizes, stepped in turn
+    sample = batched_sample()
+    ms = []
+    for g, (bx, by) in enumerate(sample):
+        mg = oracle_machine(lambda: hydro_om(CType.Float, G), (bx, by), opt=BASELINE_OPT)
+        setup_hydro(mg, bx, by, 2024 + g, np.float32)
+        ms.append(mg)
+    cells = sum(bx * by for bx, by in sample)
+
+    def batch_step(k):
+        for mg in ms:
+            mg.run("proceed", k)
+    v, runs, st, secs = _median_rate(batch_step, cells / 1e9, budget_s, 200)
+    others["batched"] = {"value": v, "unit": "Gcell-updates/s", "runs": runs,
+                         "sample": "%d steps of %d of the 256 grids (%s, %d cells) in %.1f s"
+                         % (st, len(sample), sample, cells, secs)}
+    for mg in ms:
+        mg.close()
+    # the adjoint: the CPU FP64 reference adjoint (torch autograd through the independent
+    # restatement oracle/hydro_torch.py), forward + backward per step at 512^2
+    others["adjoint"] = _cpu_adjoint(512, budget_s)
+    out["others"] = others
+    return out
+
+
+def _cpu_adjoint(n, budget_s, dt=1e-4, steps=4):
+    import numpy as np
+    import torch
+    from oracle import hydro_torch as HT
+    from paraiso_amd import inputs
+    torch.set_num_threads(int(os.environ.get("OMP_NUM_THREADS", AFFINITY_CPUS)))
+    state = inputs.kh_init(n, n, 7)
+    times = []
+    t_end = time.perf_counter() + 3 * budget_s
+    while len(times) < 3 and (not times or time.perf_counter() < t_end):
+        t = [torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in state]
+        t0 = time.perf_counter()
+        out, _ = HT.run(t, steps, (1.0 / n, 1.0 / n), 1.4, dt=dt)
+        J, _ = HT.kinetic_energy_in_square(*out[:3])
+        J.backward()
+        times.append(time.perf_counter() - t0)
+    per = float(np.median(times)) / steps
+    return {"value": n * n / per / 1e9, "unit": "Gcell-updates/s (forward + adjoint)",
+            "ms_per_step_fwd_adj": per * 1e3, "runs": [round(x, 3) for x in times],
+            "sample": "%d steps of %d^2 FP64, forward and reverse through torch autograd "
+                      "(oracle/hydro_torch.py), median of %d runs" % (steps, n, len(times))}
 
 
 def main():
@@ -505,29 +657,42 @@ def main():
                      "avg_launch_ms": r32["dominant_avg_ms"]},
         "launch_ms_per_step": r32["launch_ms_per_step"],
         "step_kernel": "proceed_pipelined",
-        "valu": None,
         "unfused": {"kernel": "proceed", "value": r32u["value"],
                     "ms_per_step": r32u["ms_per_step"],
                     "launch_ms_per_step": r32u["launch_ms_per_step"]},
     }
+    roof = out["roofline"]
+    roof["hbm"] = {"achieved": roof["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                   "frac": roof["frac"], "algorithmic_bytes_per_cell": 32}
     if (nx, ny) == (2048, 2048):
-        kname = out["roofline"]["kernel"]
-        traffic, valu = profile_figures("hydro_f32", kname)
-        out["roofline"]["traffic"] = traffic
+        traffic, valu = profile_figures("hydro_f32", roof["kernel"])
+        roof["traffic"] = traffic
+        if traffic is not None:
+            roof["traffic_source"] = ("%s: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a "
+                                      "1 GiB copy's readings" % PROFILE)
         if valu is not None:
+            # the kernel is bound by VALU issue, not HBM: the roofline is the lane-instruction
+            # rate against the CUs' issue rate, with the HBM fraction kept beside it
             t = r32["dominant_avg_ms"] * 1e-3
-            rate = valu["wave_instructions"] / t
-            out["valu"] = {"wave_instructions_per_launch": valu["wave_instructions"],
-                           "achieved_per_s": rate, "peak_per_s": VALU_PEAK,
-                           "frac": rate / VALU_PEAK,
-                           "source": "profiles/hydro2d_kh_2048x2048_fp32.md (SQ_INSTS_VALU, "
-                                     "SQ_ACTIVE_INST_VALU, SQ_INSTS_VALU_TRANS_F32)"}
+            lanes = valu["wave_instructions"] * 64
+            ops = useful_ops_per_cell("hydro_f32", "proceed_pipelined")
+            roof.update({"bound": "valu", "achieved": lanes / t / 1e12,
+                         "peak": LANE_VALU_PEAK / 1e12,
+                         "unit": "T lane-VALU instructions/s",
+                         "frac": lanes / t / LANE_VALU_PEAK,
+                         "lane_valu_per_cell": lanes / (nx * ny),
+                         "useful_ops_per_cell": ops,
+                         "useful_fp_fraction": ops["fp_arith"] / (lanes / (nx * ny)),
+                         "instructions_source": "%s: SQ_INSTS_VALU per dispatch (instruction "
+                                                "counts of this build, whose build.key the "
+                                                "profile records); the time is this run's"
+                                                % PROFILE})
             if valu["active_quad_cycles"] is not None:
                 # issue slots actually used (a transcendental holds its SIMD for two), over
                 # the slots the CUs offer during the live launch time
-                out["valu"]["transcendental_per_launch"] = valu["transcendental"]
-                out["valu"]["active_quad_cycles_per_launch"] = valu["active_quad_cycles"]
-                out["valu"]["busy"] = valu["active_quad_cycles"] / (VALU_PEAK * t)
+                roof["busy"] = valu["active_quad_cycles"] / (VALU_PEAK * t)
+            if valu.get("mix"):
+                roof["mix_per_cell"] = {k: v * 64 / (nx * ny) for k, v in valu["mix"].items()}
     if not args.no_fp64:
         m64, r64 = time_machine("hydro_f64", nx, ny, K, W, 1, np.float64, 64)
         out["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],

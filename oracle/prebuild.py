@@ -43,15 +43,51 @@ def _cases():
     for n in [(64, 48), (96, 64), (48, 32), (80, 56), (512, 512)]:
         out.append((lambda: hydro_om(CType.Double, G), n, cyc2))
         out.append((lambda: hydro_om(CType.Float, G), n, cyc2))
+    from paraiso_amd.schemes.optable import optable_om
+    for real in (CType.Float, CType.Double, CType.Int):
+        out.append((lambda real=real: optable_om(real, signum=False), (64, 48), cyc2))
     out.append((lambda: hydro_om(CType.Float, G), (1088, 544), cyc2))
     out.append((lambda: hydro_om(CType.Double, G), (544, 544), cyc2))
     return out
 
 
+BASELINE_OPT = "-O3"
+
+
+def baseline_cases():
+    """The CPU baseline's machines (the reference's CPlusPlus output restated, built -O3):
+    Hydro 2048^2 FP32, Life 1024^2, LinearWave 512^3 FP32 and a sample of the batched
+    grid sizes."""
+    from paraiso_amd import inputs
+    from paraiso_amd.schemes.life import life_om
+    from paraiso_amd.schemes.hydro import hydro_om
+    from paraiso_amd.schemes.linear_wave import linear_wave_om
+    cyc2, cyc3 = (A.Condition.Cyclic,) * 2, (A.Condition.Cyclic,) * 3
+    out = [(lambda: hydro_om(CType.Float, G), (2048, 2048), cyc2),
+           (life_om, (1024, 1024), cyc2),
+           (lambda: linear_wave_om(3, CType.Float), (512, 512, 512), cyc3)]
+    for nx, ny in batched_sample():
+        out.append((lambda: hydro_om(CType.Float, G), (nx, ny), cyc2))
+    return out
+
+
+def batched_sample(count: int = 4, seed: int = 2024, n_grids: int = 256):
+    """Grids of the batched workload the CPU baseline times: ``count`` of the 256 sizes,
+    chosen by a seeded draw."""
+    import numpy as np
+    from paraiso_amd import inputs
+    sizes = inputs.batched_sizes(n_grids, seed)
+    pick = np.random.default_rng(seed).choice(n_grids, size=count, replace=False)
+    return [tuple(int(x) for x in sizes[i]) for i in sorted(pick)]
+
+
 def prebuild(jobs: int = 4):
-    plans = [translate(Setup(n, Language.CPlusPlus, boundary=b), f()) for f, n, b in _cases()]
+    plans = [(translate(Setup(n, Language.CPlusPlus, boundary=b), f()), "-O2")
+             for f, n, b in _cases()]
+    plans += [(translate(Setup(n, Language.CPlusPlus, boundary=b), f()), BASELINE_OPT)
+              for f, n, b in baseline_cases()]
     with ThreadPoolExecutor(jobs) as ex:
-        list(ex.map(lambda p: cpp_emitter.build(p), plans))
+        list(ex.map(lambda p: cpp_emitter.build(p[0], opt=p[1]), plans))
 
 
 if __name__ == "__main__":

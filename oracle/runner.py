@@ -87,9 +87,10 @@ class CpuMachine:
             self._f("run")(self.h, k)
 
 
-def emitted_machine(plan, abs_mode: str = "fabs", openmp: bool = True) -> CpuMachine:
+def emitted_machine(plan, abs_mode: str = "fabs", openmp: bool = True,
+                    opt: str = "-O2") -> CpuMachine:
     from . import cpp_emitter
-    so = cpp_emitter.build(plan, abs_mode=abs_mode, openmp=openmp)
+    so = cpp_emitter.build(plan, abs_mode=abs_mode, opt=opt, openmp=openmp)
     return CpuMachine(so, plan.om.setup.static_values, "oracle", plan.memory_size(),
                       [k.name for k in plan.om.kernels])
 

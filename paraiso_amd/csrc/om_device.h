@@ -34,11 +34,23 @@ static inline unsigned om_resident_workgroups(unsigned ntiles, int per_cu) {
 }
 
 
-__device__ __forceinline__ int om_wrap(int x, int n) {
-  // one conditional add/subtract covers -n <= x < 2n; tiny grids fall back to %
+// Division of a tile index by a tile count d without the division expansion: m is
+// floor((2^32 - 1) / d) from the host, so umulhi(x, m) is floor(x / d) or one less for any
+// x < 2^31, and one compare corrects it (the operands are uniform: scalar instructions).
+static inline unsigned om_udiv_magic(unsigned d) { return d ? 0xFFFFFFFFu / d : 0u; }
+
+__device__ __forceinline__ unsigned om_udiv(unsigned x, unsigned d, unsigned m) {
+  const unsigned q = __umulhi(x, m);
+  return q + ((q + 1u) * d <= x ? 1u : 0u);
+}
+
+// One conditional add and subtract cover -n <= x < 2n.  `wide` (uniform: the host sets it
+// when some axis of the grid is shorter than the launch's tile plus its reach) enables the
+// remainder for coordinates further out; otherwise the branch around it is scalar.
+__device__ __forceinline__ int om_wrap(int x, int n, int wide) {
   x = (x < 0) ? x + n : x;
   x = (x >= n) ? x - n : x;
-  if ((unsigned)x >= (unsigned)n) x = ((x % n) + n) % n;
+  if (wide && (unsigned)x >= (unsigned)n) x = ((x % n) + n) % n;
   return x;
 }
 

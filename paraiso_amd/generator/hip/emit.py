@@ -116,6 +116,9 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
     if op is Op.Div and t0 is CType.Float and fast_fp32:
         # v_rcp_f32 (1 ulp) and a multiply instead of the scaled IEEE division
         return "%s * __builtin_amdgcn_rcpf(%s)" % (_p(args[0]), args[1])
+    if op is Op.Mod and out.is_float:
+        raise NotImplementedError("Mod of floating values: the reference prints C's %, "
+                                  "which takes integers only (PlanTrans.hs:678)")
     if op in infix:
         return "%s %s %s" % (_p(args[0]), infix[op], _p(args[1]))
     if op is Op.Neg:
@@ -150,13 +153,22 @@ def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: 
             # the reference calls the double libm function on float operands
             return "(float)%s((double)(%s))" % (f, args[0])
         return "%s(%s)" % (f, args[0])
-    if op is Op.Atan2:
+    if op in (Op.Atan2, Op.Pow):
+        # the reference prints atan2(x, y) and, through its fallback, pow(x, y)
+        # (PlanTrans.hs:702,706): the double functions, also on float operands
+        f = op.value.lower()
         if t0 is CType.Float:
-            return "(float)atan2((double)(%s), (double)(%s))" % (args[0], args[1])
-        return "atan2(%s, %s)" % (args[0], args[1])
+            return "(float)%s((double)(%s), (double)(%s))" % (f, args[0], args[1])
+        return "%s(%s, %s)" % (f, args[0], args[1])
     if op is Op.Cast:
         return "(%s)(%s)" % (cast_to.c_name, args[0])
-    raise NotImplementedError("operator %s has no HIP lowering" % op.value)
+    if op is Op.Mod and out.is_float:
+        raise NotImplementedError("Mod of floating values: the reference prints C's %, "
+                                  "which takes integers only (PlanTrans.hs:678)")
+    raise NotImplementedError(
+        "operator %s has no HIP lowering: the Builder never makes it, and the reference "
+        "prints it as a call to an undefined function %s (PlanTrans.hs:706)"
+        % (op.value, op.value.lower()))
 
 
 class MachineEmitter:
@@ -262,6 +274,7 @@ class MachineEmitter:
         T = self.tile
         fn = self.fn_name(ki, la.index)
         self._ki = ki
+        self._launch_reach = 0
         has_work = bool(la.targets)
         L = la.index
         # ---- argument struct
@@ -275,6 +288,9 @@ class MachineEmitter:
             fields.append("%s* __restrict__ st%d;" % (self.statics[s].dyn.ctype.c_name, s))
         fields += ["char* __restrict__ scal;", "char* __restrict__ part;", "unsigned* counter;",
                    "int n0, n1, n2;", "int t0, t1, t2;", "unsigned ntiles;",
+                   "unsigned m0, m1;  /* om_udiv reciprocals of t0 and t1 */",
+                   "int wide;  /* some axis shorter than tile plus reach: om_wrap takes the "
+                   "remainder */",
                    "unsigned nwg;   /* reduction partials written (workgroups that reduce) */"]
         struct = "struct Args_%s {\n  %s\n};\n" % (fn, "\n  ".join(fields))
 
@@ -418,6 +434,10 @@ class MachineEmitter:
         h.append("  a.scal = c->scal; a.part = c->part; a.counter = c->counter;")
         h.append("  a.n0 = c->n[0]; a.n1 = c->n[1]; a.n2 = c->n[2];")
         h.append("  a.t0 = c->tiles[0]; a.t1 = c->tiles[1]; a.t2 = c->tiles[2]; a.ntiles = c->ntiles;")
+        h.append("  a.m0 = om_udiv_magic((unsigned)c->tiles[0]); a.m1 = om_udiv_magic((unsigned)c->tiles[1]);")
+        reach = self._max_reach(sch, la)
+        h.append("  a.wide = (%s) ? 1 : 0;" % " || ".join(
+            "c->n[%d] < %d" % (ax, self.tile[ax] + reach) for ax in range(self.ndim)))
         if persist:
             h.append("  a.nwg = om_resident_workgroups(c->ntiles, %d);" % self._per_cu(la))
         else:
@@ -450,12 +470,34 @@ class MachineEmitter:
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"
 
+    def _max_reach(self, sch, la) -> int:
+        """A bound on how far outside its tile a launch's coordinates go on any axis: the
+        widest phase box plus the largest offset the launch wraps (recorded by _wrap while
+        its phases were emitted)."""
+        box = 0
+        for ph in la.phases:
+            for lo, hi in ph.box:
+                box = max(box, -lo, hi)
+        return box + max(sch.halo, self._launch_reach) + 1
+
     def _tile_coords(self, tile: str) -> List[str]:
+        """Tile origin from the tile index: one division per axis after the first, by the
+        host's reciprocal of the tile count (om_udiv), none for the last axis."""
         T = self.tile
-        return ["  const int X0 = (int)(%s %% (unsigned)a.t0) * %d;" % (tile, T[0]),
-                "  const int X1 = (int)((%s / (unsigned)a.t0) %% (unsigned)a.t1) * %d;" % (tile, T[1]),
-                "  const int X2 = (int)(%s / ((unsigned)a.t0 * (unsigned)a.t1)) * %d;" % (tile, T[2]),
-                "  (void)X0; (void)X1; (void)X2;"]
+        nd = self.ndim
+        if nd == 1:
+            return ["  const int X0 = (int)(%s) * %d, X1 = 0, X2 = 0;" % (tile, T[0]),
+                    "  (void)X0; (void)X1; (void)X2;"]
+        out = ["  const unsigned q0_ = om_udiv(%s, (unsigned)a.t0, a.m0);" % tile,
+               "  const int X0 = (int)(%s - q0_ * (unsigned)a.t0) * %d;" % (tile, T[0])]
+        if nd == 2:
+            out.append("  const int X1 = (int)q0_ * %d, X2 = 0;" % T[1])
+        else:
+            out += ["  const unsigned q1_ = om_udiv(q0_, (unsigned)a.t1, a.m1);",
+                    "  const int X1 = (int)(q0_ - q1_ * (unsigned)a.t1) * %d;" % T[1],
+                    "  const int X2 = (int)q1_ * %d;" % T[2]]
+        out.append("  (void)X0; (void)X1; (void)X2;")
+        return out
 
     def _per_cu(self, la) -> int:
         """Workgroups of this launch one CU holds at once, as LDS and work-items allow."""
@@ -900,13 +942,14 @@ class MachineEmitter:
         """Coordinate g<ax> + off on the periodic grid (direct inside interior tiles)."""
         r = self._reach[ax]
         r[0], r[1] = min(r[0], off), max(r[1], off)
+        self._launch_reach = max(self._launch_reach, abs(off))
         if self._direct:
             return "(g%d + %d)" % (ax, off)
         if self.open_axes[ax]:
             # only cells outside every store's region read past the grid; clamping keeps
             # those reads in bounds and their results are never stored
             return "om_clamp(g%d + %d, a.n%d)" % (ax, off, ax)
-        return "om_wrap(g%d + %d, a.n%d)" % (ax, off, ax)
+        return "om_wrap(g%d + %d, a.n%d, a.wide)" % (ax, off, ax)
 
     def _local_size(self, ax: int) -> str:
         m = self.lm[ax] + self.um[ax]

@@ -144,6 +144,10 @@ class Builder:
     def __pow__(self, n: int):
         return ipow(self, n)
 
+    def pow(self, y):
+        """``x ^/ y`` (Internal.hs:391): x to a rational power, ``A.Pow``."""
+        return pow_(self, y)
+
     # -- Boolean (Internal.hs:381-386): & is &&, | is ||, ~ is not
     def __and__(self, o):
         return mk_op2(Op.And, self, lift(o))
@@ -463,6 +467,20 @@ def acos(b):
 
 def atan(b):
     return mk_op1(Op.Atan, b)
+
+
+def pow_(x, y) -> Builder:
+    """``x ^/ y = mkOp2 A.Pow x (fromRational' y)`` (Internal.hs:391): the exponent is a
+    rational literal, made an immediate of x's type."""
+    if isinstance(y, Builder):
+        raise TypeError("^/ takes a rational exponent, not a builder")
+    return mk_op2(Op.Pow, lift(x), imm(y))
+
+
+def atan2(y, x) -> Builder:
+    """``A.Atan2`` (Arithmetic.hs:60), which the CPlusPlus emitter prints as ``atan2``
+    (PlanTrans.hs:702)."""
+    return mk_op2(Op.Atan2, lift(y), lift(x))
 
 
 def max_(a, b):

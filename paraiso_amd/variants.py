@@ -101,6 +101,13 @@ def _wave_adj(ndim, real):
     return mk
 
 
+def _optable(real):
+    def mk():
+        from .schemes.optable import optable_om
+        return optable_om(real)
+    return mk
+
+
 # name -> (OM factory, setup, precise)
 VARIANTS = {
     "life_i32": (_life, Setup((64, 64), Language.HIP, boundary=CYC2), True),
@@ -150,6 +157,12 @@ VARIANTS = {
     "wave3d_f64_march": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
                                                            boundary=CYC3, tile=(32, 8, 32),
                                                            march=1), True),
+    # every operator the emitters lower, one per static (checked against the oracle)
+    "optable_f32": (_optable(CType.Float), Setup((64, 48), Language.HIP, boundary=CYC2), True),
+    "optable_f32_fast": (_optable(CType.Float), Setup((64, 48), Language.HIP, boundary=CYC2),
+                         False),
+    "optable_f64": (_optable(CType.Double), Setup((64, 48), Language.HIP, boundary=CYC2), True),
+    "optable_i32": (_optable(CType.Int), Setup((64, 48), Language.HIP, boundary=CYC2), True),
     # Open (non-periodic) boundaries: ghost margins, as the reference's shipped examples
     "life_open_i32": (_life_plain, Setup((128, 128), Language.HIP, boundary=OPEN2), True),
     "life_legacy_open_i32": (_life_legacy, Setup((128, 128), Language.HIP, boundary=OPEN2), True),

@@ -17,12 +17,12 @@ CYC2 = (A.Condition.Cyclic,) * 2
 CYC3 = (A.Condition.Cyclic,) * 3
 
 
-def oracle_machine(om_factory, local_size, boundary=None, openmp=True):
+def oracle_machine(om_factory, local_size, boundary=None, openmp=True, opt="-O2"):
     """The CPU oracle (PlanTrans restatement) of a program at a compile-time size."""
     from oracle.runner import emitted_machine
     bnd = boundary or (A.Condition.Cyclic,) * len(local_size)
     plan = translate(Setup(tuple(local_size), Language.CPlusPlus, boundary=bnd), om_factory())
-    return emitted_machine(plan, openmp=openmp)
+    return emitted_machine(plan, openmp=openmp, opt=opt)
 
 
 def rel_l1(a, b):
