@@ -13,6 +13,12 @@ timed on the GPU.
     python -m paraiso_amd.autotune prepare --variant hydro_f32
     python -m paraiso_amd.autotune measure --variant hydro_f32 --size 2048 2048 --out tune.json
     python -m paraiso_amd.autotune select tune.json          # records paraiso_amd/tuned.json
+
+Two more dimensions: ``measure --depths proceed:1 proceed_x4:4 proceed_x8:8`` times the
+kernels that fuse several steps per launch (halo depth) as candidates of their own, per
+step; ``alloc-prepare`` / ``alloc-measure`` / ``alloc-select`` flip the Manifest/Delayed
+choice of single values nearest the scheduler's threshold and keep a flip that pays (the
+per-node genome of Tuning/Genetic.hs:138-146, searched locally from the tuned machine).
 """
 from __future__ import annotations
 
@@ -62,6 +68,12 @@ CANDIDATE_SETS = {
                       ((32, 32), 512), ((64, 16), 1024), ((32, 32), 1024), ((64, 32), 1024),
                       ((128, 8), 1024), ((32, 8), 256, 24), ((32, 8), 256, 96),
                       ((64, 8), 256, 96)],
+    # timed with --depths: each tile at every number of steps fused per launch
+    "life_depth": [((64, 32), 1024), ((128, 32), 1024), ((64, 64), 1024), ((32, 64), 1024),
+                   ((128, 16), 512), ((64, 16), 512)],
+    "wave_depth": [((64, 16, 256), 1024, 48, 4), ((64, 8, 256), 512, 48, 4),
+                   ((32, 8, 256), 256, 48, 4), ((64, 16, 256), 1024, 48, 2),
+                   ((32, 16, 256), 512, 48, 4)],
 }
 
 
@@ -138,13 +150,15 @@ def _setup_inputs(variant: str, m, size):
 
 def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=None,
             kernel: str = None, batch: int = 0, libs_of: str = None,
-            kernels: List[str] = None) -> List[Dict]:
+            kernels: List[str] = None, depths: List[Tuple[str, int]] = None) -> List[Dict]:
     """Device time per step of every compiled candidate; the step is ``proceed_pipelined``
     (after ``proceed_prime``) where the program has it, else ``kernel`` or ``proceed``.
     ``batch`` > 0 times a batch of that many grids of log-uniform sides on [64, 1024]
     (seed 2024) instead of one grid of ``size``; ``libs_of`` names the variant whose
     compiled candidates are timed (default: ``variant``).  ``kernels`` also times each of
-    those OM kernels on its own (``per_kernel_ms``), for a launch form per kernel."""
+    those OM kernels on its own (``per_kernel_ms``), for a launch form per kernel.
+    ``depths`` [(kernel, k)] times each kernel that fuses k steps per launch (halo depth k)
+    as another candidate, at ms / k per step: the halo depth is a tuning dimension."""
     from . import inputs, variants
     from .generator import GEN_DIR
     from .machine import Batch, Machine
@@ -172,20 +186,160 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
             step = "proceed_pipelined" if "proceed_pipelined" in m.kernels else "proceed"
             if step == "proceed_pipelined":
                 m.run("proceed_prime")
+        for kern, per in (depths or [(step, 1)]):
+            m.run(kern, warmup)
+            m.sync()
+            launches = max(1, steps // per)
+            ms = m.run_timed(kern, launches)
+            rec = {"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
+                   "kernel": kern, "steps_per_launch": per,
+                   "ms_per_step": sum(ms) / launches / per,
+                   "launch_ms": [x / launches for x in ms]}
+            if kernels:
+                rec["per_kernel_ms"] = {}
+                for k in kernels:
+                    m.run(k, 2)
+                    rec["per_kernel_ms"][k] = sum(m.run_timed(k, steps)) / steps
+            results.append(rec)
+        m.close()
+    return results
+
+
+# ---------------------------------------------------------------- per-node allocation choice
+def alloc_candidates(variant: str, kernel: str, count: int = 8) -> List[Tuple[str, int]]:
+    """Flips of the Manifest/Delayed choice nearest the scheduler's threshold for ``kernel``
+    at the variant's tuned launch form: materialised values of the least gain (flip to
+    Delayed: expanded in registers) and expanded values of the most (flip to Manifest: kept
+    in LDS).  Value ids are those of the lowered kernel."""
+    from . import variants
+    from .generator.hip.ir import lower_kernel
+    from .generator.hip.schedule import lds_budget, schedule_kernel
+    from .generator.plan import translate
+    mk, _, precise = variants.VARIANTS[variant]
+    setup = variants.setup_of(variant)
+    plan = translate(setup, mk())
+    om = plan.om
+    ki = [k.name for k in om.kernels].index(kernel)
+    f = setup.form_of(kernel)
+    lk = lower_kernel(om, ki, tuple(True for _ in range(om.ndim)), exact=not precise)
+    sch = schedule_kernel(lk, tuple(f["tile"]), lds_budget(f["threads"]), f["mat_gain"],
+                          f["march"], f.get("manifest", ()), f.get("delayed", ()))
+    mat, expanded = [], []
+    for la in sch.launches:
+        for gain, v, in_m in la.choices:
+            (mat if in_m else expanded).append((gain, v))
+    mat.sort()
+    expanded.sort(reverse=True)
+    flips = []
+    for i in range(count):
+        src = mat if i % 2 == 0 else expanded
+        if not src:
+            src = mat or expanded
+        if not src:
+            break
+        gain, v = src.pop(0)
+        flips.append(("delayed" if src is mat else "manifest", v))
+    return flips
+
+
+def _flip_name(variant: str, kernel: str, flip) -> str:
+    return "%s__alloc_%s_%s%d" % (variant, kernel, flip[0][0], flip[1])
+
+
+def _flipped_setup(variant: str, kernel: str, flip):
+    from . import variants
+    setup = variants.setup_of(variant)
+    forms = dict(setup.kernel_forms)
+    f = dict(forms.get(kernel, {}))
+    man, dly = set(f.get("manifest", ())), set(f.get("delayed", ()))
+    if flip[0] == "manifest":
+        man.add(flip[1])
+        dly.discard(flip[1])
+    else:
+        dly.add(flip[1])
+        man.discard(flip[1])
+    f["manifest"], f["delayed"] = tuple(sorted(man)), tuple(sorted(dly))
+    forms[kernel] = f
+    return setup.with_(kernel_forms=tuple(sorted(forms.items())))
+
+
+def _build_flip(variant: str, kernel: str, flip):
+    from . import variants
+    from .generator import build_machine
+    mk, _, precise = variants.VARIANTS[variant]
+    name = _flip_name(variant, kernel, flip)
+    try:
+        return flip, build_machine(_flipped_setup(variant, kernel, flip), mk(), name,
+                                   precise=precise)
+    except (ValueError, subprocess.CalledProcessError) as e:
+        print("flip %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
+        return flip, None
+
+
+def prepare_alloc(variant: str, kernel: str, count: int = 8, jobs: int = 4):
+    """Compile one machine per flip of the Manifest/Delayed choice (alloc_candidates)."""
+    from concurrent.futures import ProcessPoolExecutor
+    flips = alloc_candidates(variant, kernel, count)
+    with ProcessPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_build_flip, variant, kernel, fl) for fl in flips]
+        return [r for r in (f.result() for f in futs) if r[1] is not None]
+
+
+def measure_alloc(variant: str, kernel: str, size, count: int = 8, steps: int = 50,
+                  warmup: int = 5, step_kernel: str = None) -> List[Dict]:
+    """Device time per step of the variant as tuned and of each compiled flip."""
+    from . import variants
+    from .generator import GEN_DIR
+    from .machine import Machine
+    libs = [(None, variants.library(variant))]
+    for fl in alloc_candidates(variant, kernel, count):
+        name = _flip_name(variant, kernel, fl)
+        so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
+        if os.path.exists(so):
+            libs.append((fl, so))
+    out = []
+    for fl, so in libs:
+        print("alloc %s %s flip=%s" % (variant, kernel, fl), flush=True)
+        m = Machine(so, size)
+        _setup_inputs(variant, m, size)
+        step = step_kernel or kernel
+        if step == "proceed_pipelined":
+            m.run("proceed_prime")
         m.run(step, warmup)
         m.sync()
         ms = m.run_timed(step, steps)
-        rec = {"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
-               "kernel": step, "ms_per_step": sum(ms) / steps,
-               "launch_ms": [x / steps for x in ms]}
-        if kernels:
-            rec["per_kernel_ms"] = {}
-            for k in kernels:
-                m.run(k, 2)
-                rec["per_kernel_ms"][k] = sum(m.run_timed(k, steps)) / steps
-        results.append(rec)
+        out.append({"flip": list(fl) if fl else None, "kernel": step,
+                    "ms_per_step": sum(ms) / steps})
         m.close()
-    return results
+    return out
+
+
+def select_alloc(path: str) -> Dict:
+    """Keep the flip that ran fastest if it beat the tuned machine by more than 0.5%
+    (one step of a local search over the per-node choices; the next tuning run starts from
+    the recorded choices)."""
+    with open(path) as f:
+        data = json.load(f)
+    t = tuned()
+    for variant, rec in data.items():
+        base = next(r for r in rec["results"] if r["flip"] is None)["ms_per_step"]
+        best = min(rec["results"], key=lambda r: r["ms_per_step"])
+        entry = t.setdefault(variant, {})
+        log = entry.setdefault("alloc_search", [])
+        log.append({"kernel": rec["kernel"], "baseline_ms": round(base, 5),
+                    "flips": {("%s:%d" % tuple(r["flip"])) if r["flip"] else "none":
+                              round(r["ms_per_step"], 5) for r in rec["results"]}})
+        if best["flip"] and best["ms_per_step"] < 0.995 * base:
+            forms = entry.setdefault("kernel_forms", {})
+            f = forms.setdefault(rec["kernel"], {})
+            key = best["flip"][0]
+            other = "delayed" if key == "manifest" else "manifest"
+            f[key] = sorted(set(f.get(key, [])) | {best["flip"][1]})
+            f[other] = [v for v in f.get(other, []) if v != best["flip"][1]]
+            log[-1]["kept"] = "%s:%d" % tuple(best["flip"])
+    with open(TUNED, "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+    return t
 
 
 def select_per_kernel(path: str) -> Dict:
@@ -219,14 +373,20 @@ def select(path: str) -> Dict:
     t = tuned()
     for variant, rec in data.items():
         best = min(rec["results"], key=lambda r: r["ms_per_step"])
-        t[variant] = {"tile": best["tile"], "threads": best["threads"],
+        keep = {k: v for k, v in t.get(variant, {}).items()
+                if k in ("kernel_forms", "alloc_search", "persist")}
+        t[variant] = dict(keep)
+        t[variant].update({"tile": best["tile"], "threads": best["threads"],
                       "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
+                      "kernel": best.get("kernel"), "steps_per_launch": best.get("steps_per_launch", 1),
                       "ms_per_step": best["ms_per_step"],
                       "size": rec["size"], "steps": rec["steps"],
-                      "candidates": {"%s_n%d_g%d_m%d" % ("x".join(map(str, r["tile"])),
-                                                         r["threads"], r.get("mat_gain", 48),
-                                                         r.get("march", 0)):
-                                     round(r["ms_per_step"], 5) for r in rec["results"]}}
+                      "candidates": {"%s_n%d_g%d_m%d%s" % ("x".join(map(str, r["tile"])),
+                                                           r["threads"], r.get("mat_gain", 48),
+                                                           r.get("march", 0),
+                                                           "" if r.get("steps_per_launch", 1) == 1
+                                                           else "_k%d" % r["steps_per_launch"]):
+                                     round(r["ms_per_step"], 5) for r in rec["results"]}})
     with open(TUNED, "w") as f:
         json.dump(t, f, indent=1, sort_keys=True)
     return t
@@ -248,12 +408,42 @@ def main(argv=None):
     q.add_argument("--kernels", nargs="*", default=None,
                    help="also time each of these kernels on its own")
     q.add_argument("--set", default=None, help="a named candidate set (CANDIDATE_SETS)")
+    q.add_argument("--depths", nargs="*", default=None,
+                   help="kernel:k pairs (k steps fused per launch) timed per step as more "
+                        "candidates, e.g. proceed:1 proceed_x4:4 proceed_x8:8")
     q.add_argument("--out", required=True)
+    pa = sub.add_parser("alloc-prepare", help="compile flips of the per-node Manifest/Delayed "
+                                              "choice nearest the threshold")
+    pa.add_argument("--variant", required=True)
+    pa.add_argument("--kernel", required=True)
+    pa.add_argument("--count", type=int, default=8)
+    qa = sub.add_parser("alloc-measure")
+    qa.add_argument("--variant", required=True)
+    qa.add_argument("--kernel", required=True)
+    qa.add_argument("--size", type=int, nargs="+", required=True)
+    qa.add_argument("--count", type=int, default=8)
+    qa.add_argument("--steps", type=int, default=50)
+    qa.add_argument("--out", required=True)
+    sa = sub.add_parser("alloc-select")
+    sa.add_argument("path")
     s = sub.add_parser("select")
     s.add_argument("path")
     s.add_argument("--per-kernel", action="store_true",
                    help="record the fastest candidate of each kernel (kernel_forms)")
     a = ap.parse_args(argv)
+    if a.cmd == "alloc-prepare":
+        for fl, so in prepare_alloc(a.variant, a.kernel, a.count):
+            print(fl, so)
+        return
+    if a.cmd == "alloc-measure":
+        res = measure_alloc(a.variant, a.kernel, a.size, a.count, a.steps)
+        with open(a.out, "w") as f:
+            json.dump({a.variant: {"kernel": a.kernel, "size": a.size, "results": res}}, f,
+                      indent=1)
+        return
+    if a.cmd == "alloc-select":
+        print(json.dumps(select_alloc(a.path), indent=1))
+        return
     if a.cmd == "prepare":
         for c, so in prepare(a.variant, CANDIDATE_SETS.get(a.set)):
             print(c, so)
@@ -265,7 +455,9 @@ def main(argv=None):
                             "results": measure(variant, size, a.steps, kernel=a.kernel,
                                                batch=a.batch, libs_of=a.libs_of,
                                                kernels=a.kernels,
-                                               candidates=CANDIDATE_SETS.get(a.set))}
+                                               candidates=CANDIDATE_SETS.get(a.set),
+                                               depths=[(d.split(":")[0], int(d.split(":")[1]))
+                                                       for d in a.depths] if a.depths else None)}
             with open(a.out, "w") as f:
                 json.dump(out, f, indent=1)
     elif a.per_kernel:

@@ -31,6 +31,7 @@ from ...om.dyn import CType, Realm
 from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
+_FORCE = (frozenset(), frozenset())   # per-node Manifest/Delayed overrides of the kernel scheduled
 SCHED_DEBUG = False
 CUT_GAIN = 2.0               # unlimited materialisation (with cuts) must save this factor of work
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
@@ -91,6 +92,9 @@ class ArrayLaunch:
     tail_scalars: List[int]                 # scalar values computed in the tail
     commit: bool                            # this tail commits the scalar stores
     kernel_override: Optional[LKernel] = None   # node list with scratch reads (cut launches)
+    # the Manifest/Delayed choice space of this launch: (gain, value, materialised) for every
+    # computed value read at two or more points, gain = per-point operations LDS saves
+    choices: List[Tuple[int, int, bool]] = field(default_factory=list)
 
 
 @dataclass
@@ -114,9 +118,17 @@ def _neg(v):
 
 
 def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET,
-                    mat_gain: int = MAT_GAIN, march: int = 0) -> KernelSchedule:
+                    mat_gain: int = MAT_GAIN, march: int = 0, force_manifest=(),
+                    force_delayed=()) -> KernelSchedule:
     """``march`` > 0: workgroups stream along the last axis, ``march`` planes per step, and
-    keep each materialised value as a ring of planes (see the emitter)."""
+    keep each materialised value as a ring of planes (see the emitter).
+
+    ``force_manifest`` / ``force_delayed``: values (ids of the lowered kernel) whose
+    Manifest/Delayed choice is fixed rather than left to the ``mat_gain`` threshold -- the
+    per-node ``AllocationChoice`` of the reference (DecideAllocation.hs:41-43), which its
+    genetic tuner flipped (Tuning/Genetic.hs:138-146) and the autotuner here searches."""
+    global _FORCE
+    _FORCE = (frozenset(force_manifest), frozenset(force_delayed))
     nd = k.ndim
     zero = tuple(0 for _ in range(nd))
     nodes = k.nodes
@@ -268,8 +280,11 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
     target_set = set(targets)
     computed = sorted(v for v in cone if nodes[v].op in ("arith", "shift")
                       and nodes[v].realm is Realm.Array)
+    force_m, force_d = _FORCE
     hinted = set(v for v in computed if nodes[v].hint_manifest and v not in target_set
-                 and nodes[v].op == "arith")
+                 and nodes[v].op == "arith" and v not in force_d)
+    hinted |= set(v for v in computed if v in force_m and nodes[v].op == "arith"
+                  and v not in target_set)
     M: Set[int] = set(loads) | hinted
 
     def analyse(M):
@@ -432,7 +447,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             lcost = local_costs(M)
             ranked = []
             for v in computed:
-                if v in M or nodes[v].op != "arith":
+                if v in M or nodes[v].op != "arith" or v in force_d:
                     continue
                 E = sum(len(c) for c in lc.get(v, {}).values())
                 gain = (E - 1) * lcost[v]
@@ -496,7 +511,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             nonzero = any(c != zero for cs in lc.get(v, {}).values() for c in cs)
             if uses <= 1 and not nonzero:
                 M.discard(v)
-        else:
+        elif v not in force_m:
             uses = sum(len(c) for c in lc.get(v, {}).values())
             if uses <= 1:
                 M.discard(v)
@@ -516,8 +531,17 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if top <= lds_cap(reduces):
                 break
     loads_global = set(v for v in loads if v not in M)
+    # the choice space the autotuner searches: computed values read at two or more points
+    lcost = local_costs(M)
+    choices = []
+    for v in computed:
+        if nodes[v].op != "arith" or v in target_set:
+            continue
+        E = sum(len(c) for c in lc.get(v, {}).values())
+        if E >= 2 or v in M:
+            choices.append(((max(E, 2) - 1) * lcost[v], v, v in M))
     return ArrayLaunch(L, stores, reduces, targets, M, phases, lc, lds, top, loads_global,
-                       tail, commit)
+                       tail, commit, choices=sorted(choices, reverse=True))
 
 
 def _first_fit(live: List[LdsBuf], nbytes: int) -> int:

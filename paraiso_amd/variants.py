@@ -183,10 +183,17 @@ def setup_of(name: str):
     if rec is not None and "kernel_forms" in rec:
         forms = []
         for k, f in sorted(rec["kernel_forms"].items()):
-            forms.append((k, {"tile": tuple(f["tile"]), "threads": int(f["threads"]),
-                              "mat_gain": int(f.get("mat_gain", 48)),
-                              "march": int(f.get("march", 0)),
-                              "persist": bool(f.get("persist", False))}))
+            form = {}
+            if "tile" in f:
+                form.update({"tile": tuple(f["tile"]), "threads": int(f["threads"]),
+                             "mat_gain": int(f.get("mat_gain", 48)),
+                             "march": int(f.get("march", 0)),
+                             "persist": bool(f.get("persist", False))})
+            # the per-node Manifest/Delayed choices the autotuner kept
+            for key in ("manifest", "delayed"):
+                if f.get(key):
+                    form[key] = tuple(int(v) for v in f[key])
+            forms.append((k, form))
         setup = setup.with_(kernel_forms=tuple(forms))
     return setup
 

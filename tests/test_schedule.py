@@ -57,3 +57,31 @@ def test_reduction_scratch_is_left_free():
     s = schedule_kernel(k, (32, 32), lds_budget(1024), 48)
     for la in s.launches:
         assert la.lds_bytes + RED_SCRATCH * len(la.reduces) <= LDS_MAX
+
+
+def test_per_node_allocation_choices_are_honoured():
+    """The autotuner's per-node Manifest/Delayed choice (DecideAllocation.hs:41-43): a value
+    forced Delayed is expanded in registers, one forced Manifest is kept in LDS, and the
+    other values keep the threshold's choice (FP32 step at the headline's launch form)."""
+    om = pipeline_reduce(hydro_om(CType.Float, Fraction(7, 5)), "proceed")
+    plan = translate(Setup((64, 64), Language.HIP, boundary=CYC2), om)
+    ki = [k.name for k in plan.om.kernels].index("proceed_pipelined")
+    k = lower_kernel(plan.om, ki, (True, True), exact=True)
+    base = schedule_kernel(k, (64, 32), lds_budget(1024), 48)
+    assert len(base.launches) == 1
+    la = base.launches[0]
+    mat = sorted(v for g, v, m in la.choices if m)
+    exp = [v for g, v, m in la.choices if not m]
+    d, m = mat[0], exp[0]
+    s = schedule_kernel(k, (64, 32), lds_budget(1024), 48, force_manifest=[m],
+                        force_delayed=[d])
+    got = s.launches[0].mat
+    assert m in got and d not in got
+    assert got - {m} == la.mat - {d}
+
+
+def test_autotuner_flips_are_nearest_the_threshold():
+    from paraiso_amd import autotune
+    flips = autotune.alloc_candidates("hydro_f32", "proceed_pipelined", 4)
+    assert len(flips) == 4 and {f[0] for f in flips} <= {"manifest", "delayed"}
+    assert len(set(v for _, v in flips)) == 4
