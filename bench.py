@@ -661,6 +661,25 @@ def main():
                     "ms_per_step": r32u["ms_per_step"],
                     "launch_ms_per_step": r32u["launch_ms_per_step"]},
     }
+    # the rate with the state crossing PCIe: the four fields written from host memory
+    # before the run and read back after it (the C ABI also takes host buffers)
+    from paraiso_amd import inputs as _inp
+    host = _inp.kh_init(nx, ny, 1, dtype=np.float32)
+    m32.sync()
+    t0 = time.perf_counter()
+    for name, arr in zip(FIELDS, host):
+        m32.set(name, arr)
+    t_in = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    back = [m32.get(f) for f in FIELDS]
+    t_out = time.perf_counter() - t0
+    wall = r32["ms_per_step"] * K * 1e-3
+    out["pcie_inclusive"] = {"value": nx * ny * K / (wall + t_in + t_out) / 1e9,
+                             "unit": "Gcell-updates/s", "host_to_device_ms": t_in * 1e3,
+                             "device_to_host_ms": t_out * 1e3,
+                             "bytes_each_way": int(sum(a.nbytes for a in back)),
+                             "note": "the %d timed steps plus writing the four fields from "
+                                     "pageable host memory and reading them back" % K}
     roof = out["roofline"]
     roof["hbm"] = {"achieved": roof["achieved"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                    "frac": roof["frac"], "algorithmic_bytes_per_cell": 32}

@@ -199,6 +199,7 @@ class MachineEmitter:
         self.regions = regions or {}     # (kernel, "store", static) / (kernel, "reduce", node)
         self._reach = [[0, 0] for _ in range(3)]   # offsets seen by _wrap in the current phase
         self._direct = False
+        self._pf = None         # loop point whose prefetched registers stand for the loads
         self.fast_fp32 = fast_fp32
         self.batched = batched
         self.statics = statics           # list of Named
@@ -515,8 +516,7 @@ class MachineEmitter:
         out: List[str] = []
         w = out.append
         phases = la.phases
-        pf = (self.prefetch and len(phases) > 1 and phases[0].nodes
-              and all(k.nodes[v].op == "load" for v in phases[0].nodes))
+        pf = self.prefetch and len(phases) > 1 and phases[0].index == 0 and phases[0].nodes
 
         def remap(t):
             return ("const unsigned tile = ((a.ntiles & 7u) == 0u) ? ((%s) & 7u) * (a.ntiles >> 3) "
@@ -557,8 +557,8 @@ class MachineEmitter:
             if guarded:
                 w("    }")
             w("  }")
-        self._emit_prefetch(k, la, ph, w, box, W, vol, declare, load, "",
-                            lambda i: self._tile_point(box, W, vol, i, w), close)
+        self._emit_prefetch(k, la, ph, w, box, W, (vol + self.nt - 1) // self.nt, declare, load,
+                            "", lambda i: self._tile_point(box, W, vol, i, w), close)
 
     def _emit_tile_writes(self, k, la, ph, w):
         T = self.tile
@@ -568,13 +568,36 @@ class MachineEmitter:
         n_it = (vol + self.nt - 1) // self.nt
         w("  // phase %d: values %s into LDS (loaded one tile ahead)"
           % (ph.index, ",".join("n%d" % v for v in ph.nodes)))
+        computed = self._load_roots(k, la, ph) != sorted(ph.nodes)
         for i in range(n_it):
+            if computed:
+                guarded = self._tile_point(box, W, vol, i, w)
+                self._pf = i
+                for ln in self._ctx_body(k, la, list(ph.nodes), box, W, False):
+                    w("    " + ln)
+                self._pf = None
+                for v in ph.nodes:
+                    w("    s%d[p] = n%d_%s;" % (v, v, _cname((0,) * self.ndim)))
+                if guarded:
+                    w("    }")
+                w("  }")
+                continue
             w("  {")
             w("    const int p = threadIdx.x + %d;" % (i * self.nt))
             if (i + 1) * self.nt > vol:
                 w("    if (p < %d)" % vol)
             w("    { %s }" % " ".join("s%d[p] = pf%d_%d;" % (v, v, i) for v in ph.nodes))
             w("  }")
+
+    def _load_roots(self, k, la, ph) -> List[int]:
+        """The fields a load phase reads from memory: its materialised loads, and the loads
+        of the values it computes point by point from them."""
+        nodes = k.nodes
+        out = set(v for v in ph.nodes if nodes[v].op == "load")
+        for u, ctxs in la.lc.items():
+            if nodes[u].op == "load" and nodes[u].realm is Realm.Array and ph.index in ctxs:
+                out.add(u)
+        return sorted(out)
 
     def _tile_point(self, box, W, vol, i, w) -> bool:
         """Open the block of loop point p = threadIdx.x + i * NT of a block-tile phase."""
@@ -603,8 +626,7 @@ class MachineEmitter:
             raise ValueError("the tile's last extent %d is not a multiple of march %d" % (Z, R))
         inner = la.phases[:-1]
         s0 = min([(ph.box[ma][0] - ph.box[ma][1]) // R for ph in inner] + [0])
-        pf = (self.prefetch and inner and inner[0].nodes
-              and all(k.nodes[v].op == "load" for v in inner[0].nodes))
+        pf = self.prefetch and inner and inner[0].index == 0 and inner[0].nodes
         w("  // march along axis %d: %d planes per step over %d planes" % (ma, R, Z))
         if pf:
             ph = inner[0]
@@ -635,50 +657,69 @@ class MachineEmitter:
         W[self._m["ma"]] = self._m["R"]
         return box, W, W[0] * W[1] * W[2]
 
-    def _march_point(self, box, W, vol, i, step, w):
-        """Open the block of loop point p = threadIdx.x + i * NT of march step `step` (plane
-        jz of the box's leading R planes); the caller closes it with _march_close."""
+    def _march_points(self, vol) -> Tuple[int, int]:
+        """(points of one plane, loop points per plane of a thread) of a march phase whose
+        R leading planes hold ``vol`` points."""
+        pv = vol // self._m["R"]
+        return pv, (pv + self.nt - 1) // self.nt
+
+    def _march_point(self, box, W, vol, j, step, w, is_out=False):
+        """Open the block of the thread's loop point j of march step `step`: plane l of the
+        box's leading R planes and point p = threadIdx.x + i * NT of that plane, (l, i) =
+        divmod(j, points per plane).  The plane coordinate jz is uniform over the workgroup
+        and the in-plane coordinates do not depend on the step, so their index arithmetic
+        is evaluated once per kernel rather than at every step.  The caller closes the
+        block with _march_close."""
         m = self._m
         ma = m["ma"]
+        pv, per = self._march_points(vol)
+        l, i = divmod(j, per)
         w("  {")
+        w("    const int jz = %s * %d + %d;" % (step, m["R"], m["hi"] + l))
         w("    const int p = threadIdx.x + %d;" % (i * self.nt))
-        if (i + 1) * self.nt > vol:
-            w("    if (p < %d) {" % vol)
-        w("    const int l0 = p %% %d;" % W[0])
-        w("    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
-        w("    const int l2 = p / %d;" % (W[0] * W[1]))
+        guarded = (i + 1) * self.nt > pv
+        w("    if (%s%s) {" % ("p < %d && " % pv if guarded else "",
+                               "jz >= %d" % m["lo"] if not is_out else "true"))
+        stride = 1
+        for ax in range(3):
+            if ax == ma:
+                w("    const int l%d = %d;" % (ax, l))
+                continue
+            if W[ax] == 1:
+                w("    const int l%d = 0;" % ax)
+                continue
+            q = "p" if stride == 1 else "(p / %d)" % stride
+            last = stride * W[ax] >= pv
+            w("    const int l%d = %s;" % (ax, q if last else "%s %% %d" % (q, W[ax])))
+            stride *= W[ax]
         w("    (void)l0; (void)l1; (void)l2;")
-        w("    const int jz = %s * %d + %d + l%d;" % (step, m["R"], m["hi"], ma))
-        w("    if (jz >= %d) {" % m["lo"])
         for ax in range(3):
             if ax == ma:
                 w("    const int g%d = X%d + jz; (void)g%d;" % (ax, ax, ax))
             else:
                 w("    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
-        return (i + 1) * self.nt > vol
+        return False
 
     def _march_close(self, guarded, w):
         w("    }")
-        if guarded:
-            w("    }")
         w("  }")
 
     def _emit_march_loads(self, k, la, ph, w, step, declare):
         """Global loads of a load-only phase for march step `step` into registers
         pf<v>_<i> (one per loop point i of the thread), which the next ring write stores."""
         box, W, vol = self._march_box(ph)
-        self._emit_prefetch(k, la, ph, w, box, W, vol, declare, True, "for step %s" % step,
-                            lambda i: self._march_point(box, W, vol, i, step, w),
+        self._emit_prefetch(k, la, ph, w, box, W, self._m["R"] * self._march_points(vol)[1],
+                            declare, True, "for step %s" % step,
+                            lambda j: self._march_point(box, W, vol, j, step, w),
                             lambda guarded: self._march_close(guarded, w))
 
-    def _emit_prefetch(self, k, la, ph, w, box, W, vol, declare, load, what, open_point,
+    def _emit_prefetch(self, k, la, ph, w, box, W, n_it, declare, load, what, open_point,
                        close_point):
         """Loads of a load-only phase into registers pf<v>_<i>, i the thread's loop point:
         direct addressing for tiles whose reads stay inside the grid (a tile-uniform test),
         wrapped or clamped addressing for the others.  ``declare`` declares the registers
         first; with ``load`` False nothing else is emitted."""
-        n_it = (vol + self.nt - 1) // self.nt
-        roots = list(ph.nodes)
+        roots = self._load_roots(k, la, ph)
         if declare:
             for v in roots:
                 tn = k.nodes[v].ctype.c_name
@@ -690,7 +731,9 @@ class MachineEmitter:
         self._direct = False
         bodies = [self._ctx_body(k, la, roots, box, W, False)]
         conds = None
-        if any("om_wrap(" in ln or "om_clamp(" in ln for ln in bodies[0]):
+        # (a marching workgroup wraps every point: its plane coordinate is uniform and the
+        # in-plane coordinates do not change from step to step, so the wrap is hoisted)
+        if not self._m and any("om_wrap(" in ln or "om_clamp(" in ln for ln in bodies[0]):
             reach = self._reach
             self._direct = True
             bodies.insert(0, self._ctx_body(k, la, roots, box, W, False))
@@ -719,13 +762,22 @@ class MachineEmitter:
     def _emit_march_ring_writes(self, k, la, ph, w):
         """Store the registers loaded for this step into the value rings."""
         box, W, vol = self._march_box(ph)
-        n_it = (vol + self.nt - 1) // self.nt
+        n_it = self._m["R"] * self._march_points(vol)[1]
         w("  // phase %d: values %s into their rings (loaded one step ahead)"
           % (ph.index, ",".join("n%d" % v for v in ph.nodes)))
+        computed = self._load_roots(k, la, ph) != sorted(ph.nodes)
+        zero = _cname((0,) * self.ndim)
         for i in range(n_it):
             guarded = self._march_point(box, W, vol, i, "s", w)
+            if computed:
+                # values computed point by point from the prefetched fields
+                self._pf = i
+                for ln in self._ctx_body(k, la, list(ph.nodes), box, W, False):
+                    w("      " + ln)
+                self._pf = None
             for v in ph.nodes:
-                w("      s%d[%s] = pf%d_%d;" % (v, self._lds_self(la, v, W), v, i))
+                src = ("n%d_%s" % (v, zero)) if computed else ("pf%d_%d" % (v, i))
+                w("      s%d[%s] = %s;" % (v, self._lds_self(la, v, W), src))
             self._march_close(guarded, w)
 
     def _ring(self, la, v):
@@ -780,47 +832,56 @@ class MachineEmitter:
     def _emit_loop(self, k, la, ph, box, W, vol, is_out, roots, lines, w, ind):
         nodes = k.nodes
         nd = self.ndim
+        zero = tuple(0 for _ in range(nd))
+
+        def body(ind):
+            if is_out:
+                conds = ["g%d < a.n%d" % (ax, ax) for ax in range(nd)]
+                if self._m:
+                    w(ind + "    if (%s) {" % " && ".join(conds))
+                else:
+                    w(ind + "    if (!(%s)) continue;" % " && ".join(conds))
+                w(ind + "    const int addr0 = %s;" % self._addr(["g0", "g1", "g2"]))
+            for ln in lines:
+                w(ind + "    " + ln)
+            if is_out:
+                for s, v in la.stores:
+                    val = self._val_name(k, la, v, zero, roots)
+                    cond = self._region_cond((self._ki, "store", s))
+                    if cond:
+                        val = "(%s) ? %s : (%s)0" % (cond, val, nodes[v].ctype.c_name)
+                    w(ind + "    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s), val))
+                for r in la.reduces:
+                    arg = nodes[r].args[0]
+                    at = acc_type(nodes[r].payload, nodes[arg].ctype)
+                    val = "(%s)%s" % (at.c_name, self._val_name(k, la, arg, zero, roots))
+                    cond = self._region_cond((self._ki, "reduce", r))
+                    if cond:
+                        val = "(%s) ? %s : (%s)0" % (cond, val, at.c_name)
+                    w(ind + "    acc%d = %s(acc%d, %s);" % (r, acc_op(nodes[r].payload, at), r, val))
+                if self._m:
+                    w(ind + "    }")
+            else:
+                for v in ph.nodes:
+                    w(ind + "    s%d[%s] = n%d_%s;" % (v, self._lds_self(la, v, W), v, _cname(zero)))
+
+        if self._m:
+            # a march phase: the thread's points of each of the R planes, unrolled (see
+            # _march_point)
+            pv, per = self._march_points(vol)
+            for j in range(self._m["R"] * per):
+                self._march_point(box, W, vol, j, "s", lambda x: w(ind + x), is_out)
+                body(ind + "  ")
+                self._march_close(False, lambda x: w(ind + x))
+            return
         w(ind + "  for (int p = threadIdx.x; p < %d; p += %d) {" % (vol, self.nt))
         w(ind + "    const int l0 = p %% %d;" % W[0])
         w(ind + "    const int l1 = (p / %d) %% %d;" % (W[0], W[1]))
         w(ind + "    const int l2 = p / %d;" % (W[0] * W[1]))
         w(ind + "    (void)l0; (void)l1; (void)l2;")
-        m = self._m
-        if m:
-            ma = m["ma"]
-            w(ind + "    const int jz = s * %d + %d + l%d;" % (m["R"], m["hi"], ma))
-            if not is_out:
-                w(ind + "    if (jz < %d) continue;" % m["lo"])
         for ax in range(3):
-            if m and ax == m["ma"]:
-                w(ind + "    const int g%d = X%d + jz; (void)g%d;" % (ax, ax, ax))
-            else:
-                w(ind + "    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
-        if is_out:
-            conds = ["g%d < a.n%d" % (ax, ax) for ax in range(nd)]
-            w(ind + "    if (!(%s)) continue;" % " && ".join(conds))
-            w(ind + "    const int addr0 = %s;" % self._addr(["g0", "g1", "g2"]))
-        for ln in lines:
-            w(ind + "    " + ln)
-        zero = tuple(0 for _ in range(nd))
-        if is_out:
-            for s, v in la.stores:
-                val = self._val_name(k, la, v, zero, roots)
-                cond = self._region_cond((self._ki, "store", s))
-                if cond:
-                    val = "(%s) ? %s : (%s)0" % (cond, val, nodes[v].ctype.c_name)
-                w(ind + "    a.st%d[addr0] = %s;" % (self.sidx(self._ki, s), val))
-            for r in la.reduces:
-                arg = nodes[r].args[0]
-                at = acc_type(nodes[r].payload, nodes[arg].ctype)
-                val = "(%s)%s" % (at.c_name, self._val_name(k, la, arg, zero, roots))
-                cond = self._region_cond((self._ki, "reduce", r))
-                if cond:
-                    val = "(%s) ? %s : (%s)0" % (cond, val, at.c_name)
-                w(ind + "    acc%d = %s(acc%d, %s);" % (r, acc_op(nodes[r].payload, at), r, val))
-        else:
-            for v in ph.nodes:
-                w(ind + "    s%d[%s] = n%d_%s;" % (v, self._lds_self(la, v, W), v, _cname(zero)))
+            w(ind + "    const int g%d = X%d + %d + l%d; (void)g%d;" % (ax, ax, box[ax][0], ax, ax))
+        body(ind)
         w(ind + "  }")
 
     def _lds_self(self, la, v, W) -> str:
@@ -845,7 +906,8 @@ class MachineEmitter:
             return g[0]
         if self.ndim == 2:
             return "%s * a.n0 + %s" % (g[1], g[0])
-        return "(%s * a.n1 + %s) * a.n0 + %s" % (g[2], g[1], g[0])
+        # plane offset (uniform when the plane coordinate is) plus the in-plane offset
+        return "%s * (a.n1 * a.n0) + (%s * a.n0 + %s)" % (g[2], g[1], g[0])
 
     def _ctx_body(self, k: LKernel, la: ArrayLaunch, roots: List[int], box, W, is_out) -> List[str]:
         nodes = k.nodes
@@ -900,6 +962,10 @@ class MachineEmitter:
                         idx = "(%s) * %d + (%s) * %d + (%s)" % (terms[2], UW[0] * UW[1], terms[1],
                                                                 UW[0], terms[0])
                     lines.append("const %s %s = s%d[%s];" % (tn, name, v, idx))
+                elif n.op == "load" and self._pf is not None:
+                    if any(c):
+                        raise ValueError("a load phase reads field n%d at offset %s" % (v, c))
+                    lines.append("const %s %s = pf%d_%d;" % (tn, name, v, self._pf))
                 elif n.op == "load":
                     lines.append("const %s %s = a.ld%d[%s];" % (tn, name, self.sidx(self._ki, n.payload),
                                                                 self._gaddr(c, is_out)))

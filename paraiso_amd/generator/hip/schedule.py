@@ -287,6 +287,22 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                   and v not in target_set)
     M: Set[int] = set(loads) | hinted
 
+    def pointwise(v, M):
+        """v's expansion down to the values it reads from LDS (M) holds no Shift: every load
+        it reads is read at v's own point."""
+        seen, st = set(), list(nodes[v].args)
+        while st:
+            x = st.pop()
+            if x in seen or x in M:
+                continue
+            seen.add(x)
+            n = nodes[x]
+            if n.op == "shift" and n.realm is Realm.Array:
+                return False
+            if n.op == "arith" and n.realm is Realm.Array:
+                st.extend(n.args)
+        return True
+
     def analyse(M):
         # phases of materialised values: loads 0, others 1 + max phase they read
         reads: Dict[int, Set[int]] = {}
@@ -333,10 +349,12 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if v in loads:
                 continue
             # a value read only at offset 0 is available in registers at the same
-            # point of the same loop, so it may share the reader's phase
-            p = 1
+            # point of the same loop, so it may share the reader's phase; a value computed
+            # point by point from loaded fields joins the load phase itself (phase 0 then
+            # stores it in place of, or beside, the fields it reads)
+            p = 0 if pointwise(v, M) else 1
             for u, curs in reads_with_cursors(v).items():
-                if u in loads or curs != {zero}:
+                if curs != {zero}:
                     p = max(p, phase[u] + 1)
                 else:
                     p = max(p, phase[u])
@@ -369,9 +387,9 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                         if nodes[a].realm is Realm.Array:
                             st.append((a, c))
         for p in sorted(members):
-            if p == 0:
-                continue
-            expand(p, sorted(members[p]))
+            roots_p = [v for v in members[p] if v not in loads]
+            if roots_p:
+                expand(p, sorted(roots_p) if p == 0 else sorted(members[p]))
         expand(OUT, out_roots)
         for t in targets:
             if t in M:
@@ -471,6 +489,32 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                 break
         return M
 
+    def free_fields(M):
+        """Materialise, before the threshold's choice, values computed point by point from
+        loaded fields that take the place of a field in LDS (nothing else reads it): they
+        join the load phase, so no phase, barrier or LDS is added and any saving pays."""
+        phase, out_phase, members, lc = analyse(M)
+        lcost = local_costs(M)
+        top = layout(M)[6]
+        for v in computed:
+            if v in M or v in force_d or nodes[v].op != "arith" or not pointwise(v, M):
+                continue
+            E = sum(len(c) for c in lc.get(v, {}).values())
+            if E < 2 or (E - 1) * lcost[v] <= 0:
+                continue
+            trial = set(M) | {v}
+            lc_t = analyse(trial)[3]
+            freed = set(u for u in M if u in loads and not any(lc_t.get(u, {}).values()))
+            if not freed:
+                continue
+            trial -= freed
+            t_top = layout(trial)[6]
+            if t_top <= top:
+                M, top = trial, t_top
+                phase, out_phase, members, lc = analyse(M)
+                lcost = local_costs(M)
+        return M
+
     def work(M):
         """Per-point arithmetic a launch with materialised set M evaluates, summed over its
         phases' boxes (each evaluation of a value at a cursor counts once)."""
@@ -489,6 +533,7 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                 total += len(curs) * vol.get(ctx, 0)
         return total
 
+    M = free_fields(M)
     M_fit = greedy(set(M), budget, 64)
     if budget is not None and allow_free:
         # when the LDS budget stopped the greedy, compare with materialising without limit:

@@ -62,7 +62,9 @@ def test_reduction_scratch_is_left_free():
 def test_per_node_allocation_choices_are_honoured():
     """The autotuner's per-node Manifest/Delayed choice (DecideAllocation.hs:41-43): a value
     forced Delayed is expanded in registers, one forced Manifest is kept in LDS, and the
-    other values keep the threshold's choice (FP32 step at the headline's launch form)."""
+    other values keep the threshold's choice (FP32 step at the headline's launch form).  A
+    field may leave LDS: a value computed point by point from it is computed while the field
+    is loaded and kept in its place."""
     om = pipeline_reduce(hydro_om(CType.Float, Fraction(7, 5)), "proceed")
     plan = translate(Setup((64, 64), Language.HIP, boundary=CYC2), om)
     ki = [k.name for k in plan.om.kernels].index("proceed_pipelined")
@@ -77,7 +79,8 @@ def test_per_node_allocation_choices_are_honoured():
                         force_delayed=[d])
     got = s.launches[0].mat
     assert m in got and d not in got
-    assert got - {m} == la.mat - {d}
+    computed = lambda vs: set(v for v in vs if k.nodes[v].op != "load")
+    assert computed(got) - {m} == computed(la.mat) - {d}
 
 
 def test_autotuner_flips_are_nearest_the_threshold():
@@ -85,3 +88,40 @@ def test_autotuner_flips_are_nearest_the_threshold():
     flips = autotune.alloc_candidates("hydro_f32", "proceed_pipelined", 4)
     assert len(flips) == 4 and {f[0] for f in flips} <= {"manifest", "delayed"}
     assert len(set(v for _, v in flips)) == 4
+
+
+def test_pointwise_values_join_the_load_phase():
+    """A value computed point by point from loaded fields is computed while they are loaded.
+    In the two-step LinearWave launch, f1 = f0 + dt g0 (Convergence.hs:44) joins phase 0 and
+    is kept in LDS in place of f0, which nothing else reads, although it saves fewer
+    operations than the threshold asks of a value that adds a phase; g0 stays, as the update
+    of g reads it; the streaming kernel stores f1 from the prefetched registers, reading no
+    field at an offset in its load phase; and forced Delayed, f1 is expanded again."""
+    import re
+    from paraiso_amd.generator.hip.emit import MachineEmitter
+    from paraiso_amd.schemes.linear_wave import linear_wave_om
+    from paraiso_amd.transform import unroll
+    om = unroll(linear_wave_om(3, CType.Float), "proceed", 2)
+    plan = translate(Setup((64, 64, 64), Language.HIP, boundary=(A.Condition.Cyclic,) * 3), om)
+    ki = [k.name for k in plan.om.kernels].index("proceed_x2")
+    k = lower_kernel(plan.om, ki, (True,) * 3, exact=True)
+    loads = [n.id for n in k.nodes if n.op == "load" and n.realm.name == "Array"]
+    f0, g0 = loads[0], loads[1]
+    f1 = next(n.id for n in k.nodes if n.op == "arith" and f0 in n.args)
+    tile = (64, 16, 256)
+    delayed = schedule_kernel(k, tile, lds_budget(1024), 48, march=2, force_delayed=[f1])
+    assert f1 not in delayed.launches[0].mat and f0 in delayed.launches[0].mat
+    s = schedule_kernel(k, tile, lds_budget(1024), 48, march=2)
+    la = s.launches[0]
+    ph0 = la.phases[0]
+    assert ph0.index == 0 and sorted(ph0.nodes) == sorted([g0, f1])
+    assert f0 not in la.mat and f1 in la.lds
+    # f1 is stored once per point, and read at the Laplacian's seven points
+    assert len(la.lc[f1][1]) == 7
+    em = MachineEmitter("W", plan.om.setup.static_values, 3, [s], tile, fast_fp32=True,
+                        threads=1024, march=2)
+    src, _ = em.emit_launch(0, s, la)
+    ring = src[src.index("into their rings"):]
+    ring = ring[:ring.index("__syncthreads")]
+    assert re.search(r"s%d\[[^\]]*\] = n%d_0_0_0;" % (f1, f1), ring)
+    assert "a.ld" not in ring and "pf%d_0" % f0 in ring
