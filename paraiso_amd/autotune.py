@@ -73,7 +73,9 @@ CANDIDATE_SETS = {
                    ((128, 16), 512), ((64, 16), 512)],
     "wave_depth": [((64, 16, 256), 1024, 48, 4), ((64, 8, 256), 512, 48, 4),
                    ((32, 8, 256), 256, 48, 4), ((64, 16, 256), 1024, 48, 2),
-                   ((32, 16, 256), 512, 48, 4)],
+                   ((32, 16, 256), 512, 48, 4), ((64, 8, 256), 512, 48, 2),
+                   ((32, 16, 256), 512, 48, 2), ((64, 16, 128), 1024, 48, 2),
+                   ((128, 8, 256), 1024, 48, 2)],
 }
 
 

@@ -548,13 +548,23 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             if w_fit > CUT_GAIN * w_free:
                 M_fit = M_free
     M = M_fit
-    # values that ended up used once gain nothing from LDS
+    # values that ended up used once gain nothing from LDS -- except a field that the load
+    # phase reads anyway for a value it computes: kept, its one later read comes from LDS
+    # rather than from memory a second time
     phase, out_phase, members, lc = analyse(M)
+    loaded_by_phase0 = set()
+    st = [a for f in members.get(0, []) if f not in loads for a in nodes[f].args]
+    while st:
+        x = st.pop()
+        if x in loads:
+            loaded_by_phase0.add(x)
+        elif x not in M and nodes[x].op == "arith" and nodes[x].realm is Realm.Array:
+            st.extend(nodes[x].args)
     for v in sorted(M):
         if v in loads:
             uses = sum(len(c) for c in lc.get(v, {}).values())
             nonzero = any(c != zero for cs in lc.get(v, {}).values() for c in cs)
-            if uses <= 1 and not nonzero:
+            if uses <= 1 and not nonzero and not (uses == 1 and v in loaded_by_phase0):
                 M.discard(v)
         elif v not in force_m:
             uses = sum(len(c) for c in lc.get(v, {}).values())
