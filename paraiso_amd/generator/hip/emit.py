@@ -657,41 +657,63 @@ class MachineEmitter:
         W[self._m["ma"]] = self._m["R"]
         return box, W, W[0] * W[1] * W[2]
 
-    def _march_points(self, vol) -> Tuple[int, int]:
-        """(points of one plane, loop points per plane of a thread) of a march phase whose
-        R leading planes hold ``vol`` points."""
-        pv = vol // self._m["R"]
-        return pv, (pv + self.nt - 1) // self.nt
+    def _march_points(self, vol) -> Tuple[int, int, int, bool]:
+        """(points of one plane, loop points per plane of a thread, loop points of a thread,
+        by plane) of a march phase whose R leading planes hold ``vol`` points.  A thread walks
+        the planes one at a time when that issues no more than 10% more wave64 passes than
+        walking the R planes as one range; small planes (a 2-D row of a few waves) are walked
+        as one range."""
+        R = self._m["R"]
+        pv = vol // R
+        per = (pv + self.nt - 1) // self.nt
+        by_plane = R * ((pv + 63) // 64) <= 1.1 * ((vol + 63) // 64)
+        n = R * per if by_plane else (vol + self.nt - 1) // self.nt
+        return pv, per, n, by_plane
 
-    def _march_point(self, box, W, vol, j, step, w, is_out=False):
-        """Open the block of the thread's loop point j of march step `step`: plane l of the
-        box's leading R planes and point p = threadIdx.x + i * NT of that plane, (l, i) =
-        divmod(j, points per plane).  The plane coordinate jz is uniform over the workgroup
-        and the in-plane coordinates do not depend on the step, so their index arithmetic
-        is evaluated once per kernel rather than at every step.  The caller closes the
-        block with _march_close."""
+    def _march_point(self, box, W, vol, j, step, w, is_out=False, width0=None):
+        """Open the block of the thread's loop point j of march step `step`.  By plane: plane
+        l of the box's leading R planes and point p = threadIdx.x + i * NT of that plane, (l,
+        i) = divmod(j, points per plane); the plane coordinate jz is uniform over the
+        workgroup and the in-plane coordinates do not depend on the step, so their index
+        arithmetic is evaluated once per kernel rather than at every step.  As one range:
+        point p = threadIdx.x + j * NT of the R planes, plane p / (points of a plane).  With
+        ``width0`` below W[0], rows are walked W[0] points long and points at x >= width0
+        are skipped.  The caller closes the block with _march_close."""
         m = self._m
         ma = m["ma"]
-        pv, per = self._march_points(vol)
-        l, i = divmod(j, per)
+        pv, per, n, by_plane = self._march_points(vol)
         w("  {")
-        w("    const int jz = %s * %d + %d;" % (step, m["R"], m["hi"] + l))
-        w("    const int p = threadIdx.x + %d;" % (i * self.nt))
-        guarded = (i + 1) * self.nt > pv
-        w("    if (%s%s) {" % ("p < %d && " % pv if guarded else "",
-                               "jz >= %d" % m["lo"] if not is_out else "true"))
+        conds = []
+        if by_plane:
+            l, i = divmod(j, per)
+            w("    const int jz = %s * %d + %d;" % (step, m["R"], m["hi"] + l))
+            w("    const int p = threadIdx.x + %d;" % (i * self.nt))
+            if (i + 1) * self.nt > pv:
+                conds.append("p < %d" % pv)
+            lma = "%d" % l
+        else:
+            w("    const int p = threadIdx.x + %d;" % (j * self.nt))
+            w("    const int jz = %s * %d + %d + p / %d;" % (step, m["R"], m["hi"], pv))
+            if (j + 1) * self.nt > vol:
+                conds.append("p < %d" % vol)
+            lma = "p / %d" % pv
+        if not is_out:
+            conds.append("jz >= %d" % m["lo"])
         stride = 1
         for ax in range(3):
             if ax == ma:
-                w("    const int l%d = %d;" % (ax, l))
+                w("    const int l%d = %s;" % (ax, lma))
                 continue
             if W[ax] == 1:
                 w("    const int l%d = 0;" % ax)
                 continue
             q = "p" if stride == 1 else "(p / %d)" % stride
-            last = stride * W[ax] >= pv
+            last = by_plane and stride * W[ax] >= pv
             w("    const int l%d = %s;" % (ax, q if last else "%s %% %d" % (q, W[ax])))
             stride *= W[ax]
+        if width0 is not None and width0 < W[0]:
+            conds.append("l0 < %d" % width0)
+        w("    if (%s) {" % (" && ".join(conds) if conds else "true"))
         w("    (void)l0; (void)l1; (void)l2;")
         for ax in range(3):
             if ax == ma:
@@ -708,7 +730,7 @@ class MachineEmitter:
         """Global loads of a load-only phase for march step `step` into registers
         pf<v>_<i> (one per loop point i of the thread), which the next ring write stores."""
         box, W, vol = self._march_box(ph)
-        self._emit_prefetch(k, la, ph, w, box, W, self._m["R"] * self._march_points(vol)[1],
+        self._emit_prefetch(k, la, ph, w, box, W, self._march_points(vol)[2],
                             declare, True, "for step %s" % step,
                             lambda j: self._march_point(box, W, vol, j, step, w),
                             lambda guarded: self._march_close(guarded, w))
@@ -762,7 +784,7 @@ class MachineEmitter:
     def _emit_march_ring_writes(self, k, la, ph, w):
         """Store the registers loaded for this step into the value rings."""
         box, W, vol = self._march_box(ph)
-        n_it = self._m["R"] * self._march_points(vol)[1]
+        n_it = self._march_points(vol)[2]
         w("  // phase %d: values %s into their rings (loaded one step ahead)"
           % (ph.index, ",".join("n%d" % v for v in ph.nodes)))
         computed = self._load_roots(k, la, ph) != sorted(ph.nodes)
@@ -866,11 +888,19 @@ class MachineEmitter:
                     w(ind + "    s%d[%s] = n%d_%s;" % (v, self._lds_self(la, v, W), v, _cname(zero)))
 
         if self._m:
-            # a march phase: the thread's points of each of the R planes, unrolled (see
-            # _march_point)
-            pv, per = self._march_points(vol)
-            for j in range(self._m["R"] * per):
-                self._march_point(box, W, vol, j, "s", lambda x: w(ind + x), is_out)
+            # a march phase: the thread's loop points, unrolled (see _march_point).  A phase
+            # whose rows do not fill whole waves walks them as long as the widest LDS rows it
+            # reads, so a wave's reads of those rows are consecutive words (no bank conflict
+            # where a wave crosses from one row to the next)
+            Wr = list(W)
+            if W[0] % 64:
+                T0 = self.tile[0]
+                pitch = [T0 + b.box[0][1] - b.box[0][0] for v, b in la.lds.items()
+                         if ph.index in la.lc.get(v, {}) or (is_out and OUT in la.lc.get(v, {}))]
+                Wr[0] = max([W[0]] + pitch)
+            volr = vol // W[0] * Wr[0]
+            for j in range(self._march_points(volr)[2]):
+                self._march_point(box, Wr, volr, j, "s", lambda x: w(ind + x), is_out, W[0])
                 body(ind + "  ")
                 self._march_close(False, lambda x: w(ind + x))
             return
