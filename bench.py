@@ -452,6 +452,10 @@ def bench_wave(n=512, steps=300, every=20):
     m.run(fwd, steps // k)
     m.sync()
     t_fwd = time.perf_counter() - t0
+    # once untimed, so that every snapshot slot has been written before (the first write to
+    # fresh device memory is slower than the steady state an optimisation loop sees)
+    reset()
+    m.adjoint_forward(fwd, steps // k, every // k)
     reset()
     t0 = time.perf_counter()
     m.adjoint_forward(fwd, steps // k, every // k)

@@ -157,6 +157,11 @@ VARIANTS = {
     "wave3d_f64_march": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
                                                            boundary=CYC3, tile=(32, 8, 32),
                                                            march=1), True),
+    # block tiles with the resident tile loop, whose load phase computes the half step f1
+    # from the next tile's prefetched fields, checked bit for bit
+    "wave3d_f64_persist": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
+                                                             boundary=CYC3, tile=(16, 8, 8),
+                                                             threads=512, persist=True), True),
     # every operator the emitters lower, one per static (checked against the oracle)
     "optable_f32": (_optable(CType.Float), Setup((64, 48), Language.HIP, boundary=CYC2), True),
     "optable_f32_fast": (_optable(CType.Float), Setup((64, 48), Language.HIP, boundary=CYC2),

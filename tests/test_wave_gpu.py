@@ -23,7 +23,11 @@ def params(m, c=1.0, dx=1.0, dt=0.5):
 
 @pytest.mark.parametrize("variant,n", [("wave3d_f64", (16, 12, 10)),
                                        ("wave3d_f64_march", (16, 12, 10)),
-                                       ("wave3d_f64_march", (40, 9, 70))])
+                                       ("wave3d_f64_march", (40, 9, 70)),
+                                       ("wave3d_f64_persist", (16, 12, 10)),
+                                       # more tiles than resident workgroups: each takes
+                                       # several, loading the next one's fields early
+                                       ("wave3d_f64_persist", (128, 96, 112))])
 def test_wave3d_f64_bit_exact_vs_oracle(require_gpu, variant, n):
     print("wave case %s %s" % (variant, n), flush=True)
     rng = np.random.default_rng(3)
@@ -102,7 +106,8 @@ def test_wave3d_two_step_adjoint_equals_two_adjoint_steps(require_gpu, variant):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant,k", [("wave3d_f64", 2), ("wave3d_f64_march", 2)])
+@pytest.mark.parametrize("variant,k", [("wave3d_f64", 2), ("wave3d_f64_march", 2),
+                                       ("wave3d_f64_persist", 2)])
 def test_wave3d_several_steps_per_launch(require_gpu, variant, k):
     """proceed_x<k> (k leapfrog steps per launch, halo k) stores what k x proceed stores."""
     n = (40, 9, 70)
