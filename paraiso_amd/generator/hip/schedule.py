@@ -490,9 +490,9 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         return M
 
     def free_fields(M):
-        """Materialise, before the threshold's choice, values computed point by point from
-        loaded fields that take the place of a field in LDS (nothing else reads it): they
-        join the load phase, so no phase, barrier or LDS is added and any saving pays."""
+        """Materialise values computed point by point from loaded fields that take the place
+        of a field in LDS (nothing else reads it): they join the load phase, so no phase,
+        barrier or LDS is added and any saving pays."""
         phase, out_phase, members, lc = analyse(M)
         lcost = local_costs(M)
         top = layout(M)[6]
@@ -505,10 +505,15 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             trial = set(M) | {v}
             lc_t = analyse(trial)[3]
             freed = set(u for u in M if u in loads and not any(lc_t.get(u, {}).values()))
+            if SCHED_DEBUG:
+                print("free_fields L%d: value %d gain %d frees %s" % (L, v, (E - 1) * lcost[v],
+                                                                    sorted(freed)))
             if not freed:
                 continue
             trial -= freed
             t_top = layout(trial)[6]
+            if SCHED_DEBUG:
+                print("free_fields L%d: value %d LDS %d -> %d" % (L, v, top, t_top))
             if t_top <= top:
                 M, top = trial, t_top
                 phase, out_phase, members, lc = analyse(M)
