@@ -51,10 +51,9 @@ typedef struct pr_launch_ctx {
   void* const* next;       /* per static: buffer a store writes (stored arrays) or NULL */
   char* scal;              /* device scalar block: statics, slots, counters */
   char* part;              /* this launch's reduction partials */
-  unsigned* counter;       /* an 8-byte device word per launch, zero before it: a launch
-                              whose one reduction is an integer sum finishes it in its last
-                              workgroup through this word and leaves it zero; NULL makes it
-                              use its one-workgroup finish kernel instead */
+  unsigned* counter;       /* a zeroed device word per launch, reserved: the generated
+                              kernels finish reductions in a separate one-workgroup kernel
+                              and do not use it */
   int n[3];                /* grid extent per axis (1 for absent axes) */
   int tiles[3];            /* tiles per axis */
   unsigned ntiles;         /* product of tiles[] */

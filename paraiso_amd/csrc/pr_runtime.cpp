@@ -1090,10 +1090,12 @@ extern "C" int pr_run_noalloc(const pr_machine_desc* d, const int* local_size, i
   }
   if (cells >= (1LL << 31))
     return noalloc_fail(err, err_len, PR_E_INVALID, "grid of %lld cells exceeds 2^31 - 1", cells);
-  // scratch layout: scalar block (the statics and slots at pr_create's offsets, then its
-  // per-launch words, unused here), this kernel's partials, scratch arrays
-  size_t n_counter = 0;
-  for (int kk = 0; kk < d->n_kernel; ++kk) n_counter += (size_t)d->kernels[kk].n_launch;
+  // scratch layout: scalar block, this kernel's partials, scratch arrays
+  size_t n_counter = 0, counter0 = 0;
+  for (int kk = 0; kk < d->n_kernel; ++kk) {
+    if (kk == k) counter0 = n_counter;
+    n_counter += (size_t)d->kernels[kk].n_launch;
+  }
   const size_t scal_bytes = round256(8 * ((size_t)ns + d->n_slot + n_counter));
   size_t need = scal_bytes;
   size_t part_off[PR_NOALLOC_MAX_LAUNCHES];
@@ -1193,9 +1195,7 @@ extern "C" int pr_run_noalloc(const pr_machine_desc* d, const int* local_size, i
       ctx.next = next;
       ctx.scal = scal;
       ctx.part = base + part_off[l];
-      // no counter word: a launch that could finish its reduction in its last workgroup
-      // uses the finish kernel instead, so nothing here must start at zero
-      ctx.counter = nullptr;
+      ctx.counter = reinterpret_cast<unsigned*>(scal + 8 * ((size_t)ns + d->n_slot + counter0 + l));
       for (int a = 0; a < 3; ++a) {
         ctx.n[a] = n[a];
         ctx.tiles[a] = tiles[l][a];

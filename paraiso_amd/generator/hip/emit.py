@@ -278,9 +278,6 @@ class MachineEmitter:
         self._launch_reach = 0
         has_work = bool(la.targets)
         L = la.index
-        # an integer-sum reduction may be finished by the last workgroup (see epilogue below)
-        fold_ok = (has_work and len(la.reduces) == 1 and nodes[la.reduces[0]].payload is ReduceOp.Sum
-                   and acc_type(ReduceOp.Sum, nodes[nodes[la.reduces[0]].args[0]].ctype) is CType.Int)
         # ---- argument struct
         loads = sorted(set(self.sidx(ki, nodes[v].payload) for v in la.lc if nodes[v].op == "load"
                            and nodes[v].realm is Realm.Array))
@@ -296,8 +293,6 @@ class MachineEmitter:
                    "int wide;  /* some axis shorter than tile plus reach: om_wrap takes the "
                    "remainder */",
                    "unsigned nwg;   /* reduction partials written (workgroups that reduce) */"]
-        if fold_ok:
-            fields.append("int fold;  /* the last workgroup finishes the reduction (no finish kernel) */")
         struct = "struct Args_%s {\n  %s\n};\n" % (fn, "\n  ".join(fields))
 
         decls: List[str] = []
@@ -346,26 +341,6 @@ class MachineEmitter:
         # kernel's scalar stores (a launch with none of these has no finish kernel)
         needs_tail = bool(la.reduces) or bool(la.tail_scalars) or (la.commit and bool(sch.scalar_stores))
 
-        # thread 0's part of the finish: scalars computed from the reductions (fr<id>), and
-        # the commit of the kernel's scalar stores
-        tail_body = []
-        for v in la.tail_scalars:
-            n = nodes[v]
-            expr = "(%s)(fr%d)" % (n.ctype.c_name, v) if n.op == "reduce" else self.scalar_expr(k, v)
-            tail_body.append("    const %s u%d = %s;" % (n.ctype.c_name, v, expr))
-            tail_body.append("    *(%s*)(a.scal + %d) = u%d;" % (n.ctype.c_name, self.slot_off(ki, v), v))
-        if la.commit:
-            for s_, v in sch.scalar_stores:
-                tail_body.append("    *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s_, v))
-        # a launch whose one reduction is an integer sum can finish without the finish kernel:
-        # every workgroup adds its partial (biased by 2^31, so it is non-negative and below
-        # 2^32) and a count of one to a single 64-bit word, partials in bits 0-47 and
-        # arrivals in bits 48-63, so one atomic carries both, and the workgroup that arrives
-        # last has the exact sum (mod 2^32, as the wrapping sum) without reading what the
-        # others wrote; it runs the finish and clears the word for the next launch.  Up to
-        # 65,535 workgroups; the host launcher turns it off (a.fold = 0) beyond, and when the
-        # caller passes no counter word
-
         def epilogue(index):
             e = []
             for j, r in enumerate(la.reduces):
@@ -373,23 +348,6 @@ class MachineEmitter:
                 at = acc_type(nodes[r].payload, a_ct)
                 e.append("  { const %s br = om_block_reduce<%d>(acc%d, %s, om_scr_%s);" % (
                     at.c_name, self.nt, r, acc_op(nodes[r].payload, at), at.c_name))
-                if fold_ok:
-                    e.append("    if (a.fold) {")
-                    e.append("      if (threadIdx.x == 0) {")
-                    e.append("        const unsigned long long u = (unsigned long long)((unsigned)br ^ 0x80000000u);")
-                    e.append("        const unsigned long long old = atomicAdd((unsigned long long*)a.counter, "
-                             "(1ull << 48) + u);")
-                    e.append("        if ((unsigned)(old >> 48) == a.nwg - 1u) {")
-                    e.append("          const unsigned long long tot = (old & 0xFFFFFFFFFFFFull) + u;")
-                    e.append("          const int fr%d = (int)(unsigned)(tot - (unsigned long long)a.nwg * "
-                             "0x80000000ull);" % r)
-                    e.extend("      " + ln for ln in tail_body)
-                    e.append("          *(unsigned long long*)a.counter = 0ull;")
-                    e.append("        }")
-                    e.append("      }")
-                    e.append("    } else if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + %s] = br; }" % (
-                        at.c_name, j, index))
-                    continue
                 e.append("    if (threadIdx.x == 0) ((%s*)a.part)[%d * a.ntiles + %s] = br; }" % (
                     at.c_name, j, index))
             return e if has_work else []
@@ -414,7 +372,14 @@ class MachineEmitter:
                                               acc_init(nodes[r].payload, at),
                                               acc_op(nodes[r].payload, at), at.c_name))
             f("  if (threadIdx.x == 0) {")
-            fin.extend(tail_body)
+            for v in la.tail_scalars:
+                n = nodes[v]
+                expr = "(%s)(fr%d)" % (n.ctype.c_name, v) if n.op == "reduce" else self.scalar_expr(k, v)
+                f("    const %s u%d = %s;" % (n.ctype.c_name, v, expr))
+                f("    *(%s*)(a.scal + %d) = u%d;" % (n.ctype.c_name, self.slot_off(ki, v), v))
+            if la.commit:
+                for s_, v in sch.scalar_stores:
+                    f("    *(%s*)(a.scal + %d) = u%d;" % (nodes[v].ctype.c_name, 8 * s_, v))
             f("  }")
         fin_decls = ["  __shared__ %s om_scr_%s[%d];" % (at.c_name, at.c_name, self.nt // 64)
                      for at in sorted(acc_types, key=lambda c: c.value)]
@@ -478,12 +443,9 @@ class MachineEmitter:
             h.append("  a.nwg = om_resident_workgroups(c->ntiles, %d);" % self._per_cu(la))
         else:
             h.append("  a.nwg = c->ntiles;")
-        if fold_ok:
-            h.append("  a.fold = (c->counter != nullptr && a.nwg < 65536u) ? 1 : 0;")
         h.append("}")
         h.append("static void pack_%s(const pr_launch_ctx* c, void* out) {" % fn)
-        h.append("  Args_%s a; fill_%s(c, a); a.nwg = c->ntiles;%s memcpy(out, &a, sizeof a);"
-                 % (fn, fn, " a.fold = 0;" if fold_ok else ""))
+        h.append("  Args_%s a; fill_%s(c, a); a.nwg = c->ntiles; memcpy(out, &a, sizeof a);" % (fn, fn))
         h.append("}")
         h.append("static int launch_batched_%s(const pr_batch_ctx* bc) {" % fn)
         if not self.batched:
@@ -503,8 +465,8 @@ class MachineEmitter:
             h.append("  hipLaunchKernelGGL(%s, dim3(%s), dim3(%d), 0, (hipStream_t)c->stream, a);"
                      % (fn, "a.nwg" if persist else "c->ntiles", self.nt))
         if needs_tail:
-            h.append("  %shipLaunchKernelGGL(%s_fin, dim3(1), dim3(%d), 0, (hipStream_t)c->stream, a);"
-                     % ("if (!a.fold) " if fold_ok else "", fn, self.nt))
+            h.append("  hipLaunchKernelGGL(%s_fin, dim3(1), dim3(%d), 0, (hipStream_t)c->stream, a);"
+                     % (fn, self.nt))
         h.append("  return hipPeekAtLastError() == hipSuccess ? 0 : -1;")
         h.append("}")
         return kernel_src, "\n".join(h) + "\n"
