@@ -71,6 +71,12 @@ CANDIDATE_SETS = {
     # timed with --depths: each tile at every number of steps fused per launch
     "life_depth": [((64, 32), 1024), ((128, 32), 1024), ((64, 64), 1024), ((32, 64), 1024),
                    ((128, 16), 512), ((64, 16), 512)],
+    # the headline step with and without the resident tile loop (two workgroups per CU up
+    # to 512 work-items and 80 KiB of LDS, one above)
+    "headline_persist": [((64, 32), 1024, 48, 0, True), ((64, 32), 1024, 48, 0, False),
+                         ((32, 32), 512, 48, 0, True), ((64, 16), 512, 48, 0, True),
+                         ((32, 16), 512, 24, 0, True), ((32, 64), 1024, 48, 0, True),
+                         ((128, 16), 1024, 48, 0, True), ((64, 32), 512, 48, 0, True)],
     "wave_depth": [((64, 16, 256), 1024, 48, 4), ((64, 8, 256), 512, 48, 4),
                    ((32, 8, 256), 256, 48, 4), ((64, 16, 256), 1024, 48, 2),
                    ((32, 16, 256), 512, 48, 4), ((64, 8, 256), 512, 48, 2),
@@ -87,27 +93,31 @@ def tuned() -> Dict:
 
 
 def candidate_name(variant: str, tile, threads: int = 256, mat_gain: int = 48,
-                   march: int = 0) -> str:
+                   march: int = 0, persist=None) -> str:
     g = "" if mat_gain == 48 else "_g%d" % mat_gain
     mr = "_m%d" % march if march else ""
-    return "%s__t%s_n%d%s%s" % (variant, "x".join(str(t) for t in tile), threads, g, mr)
+    pr = "" if persist is None else ("_p" if persist else "_np")
+    return "%s__t%s_n%d%s%s%s" % (variant, "x".join(str(t) for t in tile), threads, g, mr, pr)
 
 
 def _unpack(c):
-    """(tile, threads[, mat_gain[, march]]) -> (tile, threads, mat_gain, march)."""
+    """(tile, threads[, mat_gain[, march[, persist]]]) -> (tile, threads, mat_gain, march,
+    persist); persist None keeps the variant's own choice of the resident tile loop."""
     return (tuple(c[0]), int(c[1]), int(c[2]) if len(c) > 2 else 48,
-            int(c[3]) if len(c) > 3 else 0)
+            int(c[3]) if len(c) > 3 else 0, bool(c[4]) if len(c) > 4 else None)
 
 
 def _build_candidate(variant: str, c):
     from . import variants
     from .generator import build_machine
     mk, setup, precise = variants.VARIANTS[variant]
-    tile, nt, gain, march = _unpack(c)
-    name = candidate_name(variant, tile, nt, gain, march)
+    tile, nt, gain, march, persist = _unpack(c)
+    name = candidate_name(variant, tile, nt, gain, march, persist)
+    form = dict(tile=tile, threads=nt, mat_gain=gain, march=march)
+    if persist is not None:
+        form["persist"] = persist
     try:
-        return c, build_machine(setup.with_(tile=tile, threads=nt, mat_gain=gain, march=march),
-                                mk(), name, precise=precise)
+        return c, build_machine(setup.with_(**form), mk(), name, precise=precise)
     except (ValueError, subprocess.CalledProcessError) as e:
         print("candidate %s skipped: %s" % (name, str(e).splitlines()[0]), file=sys.stderr)
         return c, None
@@ -168,13 +178,13 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
     cands = candidates or CANDIDATES[setup.ndim]
     results = []
     for c in cands:
-        tile, nt, gain, march = _unpack(c)
-        name = candidate_name(libs_of or variant, tile, nt, gain, march)
+        tile, nt, gain, march, persist = _unpack(c)
+        name = candidate_name(libs_of or variant, tile, nt, gain, march, persist)
         so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
         if not os.path.exists(so):
             continue
-        print("tune %s tile=%s threads=%d mat_gain=%d march=%d" % (variant, tile, nt, gain, march),
-              flush=True)
+        print("tune %s tile=%s threads=%d mat_gain=%d march=%d persist=%s"
+              % (variant, tile, nt, gain, march, persist), flush=True)
         if batch:
             sizes = [tuple(int(x) for x in sz) for sz in inputs.batched_sizes(batch, 2024)]
             m = Batch(so, sizes)
@@ -194,7 +204,7 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
             launches = max(1, steps // per)
             ms = m.run_timed(kern, launches)
             rec = {"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
-                   "kernel": kern, "steps_per_launch": per,
+                   "persist": persist, "kernel": kern, "steps_per_launch": per,
                    "ms_per_step": sum(ms) / launches / per,
                    "launch_ms": [x / launches for x in ms]}
             if kernels:
@@ -378,16 +388,20 @@ def select(path: str) -> Dict:
         keep = {k: v for k, v in t.get(variant, {}).items()
                 if k in ("kernel_forms", "alloc_search", "persist")}
         t[variant] = dict(keep)
+        if best.get("persist") is not None:
+            t[variant]["persist"] = best["persist"]
         t[variant].update({"tile": best["tile"], "threads": best["threads"],
                       "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
                       "kernel": best.get("kernel"), "steps_per_launch": best.get("steps_per_launch", 1),
                       "ms_per_step": best["ms_per_step"],
                       "size": rec["size"], "steps": rec["steps"],
-                      "candidates": {"%s_n%d_g%d_m%d%s" % ("x".join(map(str, r["tile"])),
-                                                           r["threads"], r.get("mat_gain", 48),
-                                                           r.get("march", 0),
-                                                           "" if r.get("steps_per_launch", 1) == 1
-                                                           else "_k%d" % r["steps_per_launch"]):
+                      "candidates": {"%s_n%d_g%d_m%d%s%s" % ("x".join(map(str, r["tile"])),
+                                                             r["threads"], r.get("mat_gain", 48),
+                                                             r.get("march", 0),
+                                                             "" if r.get("persist") is None
+                                                             else ("_p" if r["persist"] else "_np"),
+                                                             "" if r.get("steps_per_launch", 1) == 1
+                                                             else "_k%d" % r["steps_per_launch"]):
                                      round(r["ms_per_step"], 5) for r in rec["results"]}})
     with open(TUNED, "w") as f:
         json.dump(t, f, indent=1, sort_keys=True)
