@@ -125,3 +125,16 @@ def test_pointwise_values_join_the_load_phase():
     ring = ring[:ring.index("__syncthreads")]
     assert re.search(r"s%d\[[^\]]*\] = n%d_0_0_0;" % (f1, f1), ring)
     assert "a.ld" not in ring and "pf%d_0" % f0 in ring
+
+
+def test_resident_loop_is_a_candidate_dimension():
+    """A candidate may fix the resident tile loop (a fifth element); without it the variant's
+    own choice stands and the candidate keeps its earlier name."""
+    from paraiso_amd import autotune
+    assert autotune._unpack(((64, 32), 1024)) == ((64, 32), 1024, 48, 0, None)
+    assert autotune._unpack(((64, 32), 1024, 48, 0, True))[4] is True
+    assert autotune.candidate_name("v", (64, 32), 1024) == "v__t64x32_n1024"
+    assert autotune.candidate_name("v", (64, 32), 1024, 48, 0, True) == "v__t64x32_n1024_p"
+    assert autotune.candidate_name("v", (64, 32), 1024, 48, 0, False) == "v__t64x32_n1024_np"
+    for c in autotune.CANDIDATE_SETS["headline_persist"]:
+        assert autotune._unpack(c)[4] is not None
