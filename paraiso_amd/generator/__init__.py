@@ -28,8 +28,12 @@ GEN_DIR = os.path.join(PKG_DIR, "_gen")
 
 DEFAULT_TILES = {1: (256,), 2: (64, 8), 3: (64, 4, 4)}
 
+# the max-ILP instruction scheduler: with the same instructions (results are unchanged) it
+# ran every workload here 0.6-2% faster than the default (Hydro 2048^2 FP32 and FP64, the
+# batched Hydro, LinearWave 512^3 two steps per launch, Life 1024^2)
 HIPCC_BASE = ["hipcc", "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-mllvm", "--amdgpu-sched-strategy=max-ilp",
               "-Wno-unused-variable", "-Wno-unused-result", "-Wno-unused-value"]
 
 
