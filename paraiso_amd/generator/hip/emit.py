@@ -80,33 +80,55 @@ def _split_const(x: Fraction) -> Tuple[np.float32, np.float32]:
     return hi, fraction_to_float32(x - Fraction(float(hi)))
 
 
-def _const_product(x: str, c: Fraction) -> str:
+def _const_product(x: str, c: Fraction, compensate=None) -> str:
     """x * c in FP32 for an exact rational c: a plain product when c is a binary32 value,
     else fma(x, hi, x * lo), which rounds x * c correctly but for near-ties.  A product by
     the rounded constant instead (0.4f for 2/5, 1.4f for 7/5) is off by the constant's own
-    rounding error, the same sign in every cell and every step."""
+    rounding error, the same sign in every cell and every step.  ``compensate`` (a set of
+    constants, or None for all) limits the compensated form to the constants it holds; the
+    others are multiplied by their nearest binary32 value."""
     hi, lo = _split_const(c)
-    if lo == 0:
+    if lo == 0 or (compensate is not None and c not in compensate):
         return "%s * %s" % (_p(x), c_literal(hi, CType.Float))
     return "__builtin_fmaf(%s, %s, %s * %s)" % (x, c_literal(hi, CType.Float), _p(x),
                                                c_literal(lo, CType.Float))
 
 
+def inverse_pairs(k: LKernel) -> Set[Fraction]:
+    """The constants a kernel multiplies by (a product, or a division by the reciprocal)
+    whose reciprocal it also multiplies by: c and 1/c convert a value forth and back (the
+    pressure and the energy of a gas through gamma - 1), and unless x * c is correctly
+    rounded the round trip drifts by the same factor every step.  Other constants
+    (gamma in a sound speed, a wave-speed estimate) enter no round trip and are multiplied
+    by their nearest binary32 value."""
+    mult = set()
+    for n in k.nodes:
+        if n.op != "arith" or n.payload[0] not in (Op.Mul, Op.Div) or n.ctype is not CType.Float:
+            continue
+        a, b = (k.exact.get(x) for x in n.args[:2])
+        if n.payload[0] is Op.Mul:
+            mult.update(c for c in (a, b) if c is not None and c != 0)
+        elif b is not None and b != 0:
+            mult.add(1 / b)
+    return set(c for c in mult if 1 / c in mult)
+
+
 def device_arith(op: Op, args: List[str], cast_to, arg_types: List[CType], out: CType,
-                 fast_fp32: bool = False, consts=None) -> str:
+                 fast_fp32: bool = False, consts=None, compensate=None) -> str:
     """C expression of one operation.  ``consts`` holds, per argument, the exact rational
-    value of a constant argument (exact-folding builds) or None."""
+    value of a constant argument (exact-folding builds) or None; ``compensate`` as for
+    _const_product."""
     consts = consts or [None] * len(args)
     if fast_fp32 and out is CType.Float and op in (Op.Mul, Op.Div) and any(
             c is not None for c in consts):
         if op is Op.Mul and consts[1] is not None and consts[0] is None:
-            return _const_product(args[0], consts[1])
+            return _const_product(args[0], consts[1], compensate)
         if op is Op.Mul and consts[0] is not None and consts[1] is None:
-            return _const_product(args[1], consts[0])
+            return _const_product(args[1], consts[0], compensate)
         if op is Op.Div and consts[1] is not None and consts[0] is None and consts[1] != 0:
-            return _const_product(args[0], 1 / consts[1])
+            return _const_product(args[0], 1 / consts[1], compensate)
         if op is Op.Div and consts[0] is not None and consts[1] is None:
-            return _const_product("__builtin_amdgcn_rcpf(%s)" % args[1], consts[0])
+            return _const_product("__builtin_amdgcn_rcpf(%s)" % args[1], consts[0], compensate)
     infix = {Op.Add: "+", Op.Sub: "-", Op.Mul: "*", Op.Div: "/", Op.Mod: "%",
              Op.And: "&&", Op.Or: "||", Op.EQ: "==", Op.NE: "!=", Op.LT: "<",
              Op.LE: "<=", Op.GT: ">", Op.GE: ">="}
@@ -200,6 +222,7 @@ class MachineEmitter:
         self._reach = [[0, 0] for _ in range(3)]   # offsets seen by _wrap in the current phase
         self._direct = False
         self._pf = None         # loop point whose prefetched registers stand for the loads
+        self._comp = None       # constants whose products are compensated (None: all)
         self.fast_fp32 = fast_fp32
         self.batched = batched
         self.statics = statics           # list of Named
@@ -275,6 +298,7 @@ class MachineEmitter:
         T = self.tile
         fn = self.fn_name(ki, la.index)
         self._ki = ki
+        self._comp = inverse_pairs(k) if self.fast_fp32 else None
         self._launch_reach = 0
         has_work = bool(la.targets)
         L = la.index
@@ -1021,7 +1045,7 @@ class MachineEmitter:
                     args = ["n%d_%s" % (a_, _cname(c)) for a_ in n.args]
                     lines.append("const %s %s = %s;" % (tn, name, device_arith(
                         op, args, cast_to, [nodes[a_].ctype for a_ in n.args], n.ctype,
-                        self.fast_fp32, [k.exact.get(a_) for a_ in n.args])))
+                        self.fast_fp32, [k.exact.get(a_) for a_ in n.args], self._comp)))
                 else:
                     raise NotImplementedError(n.op)
         return lines
