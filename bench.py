@@ -157,9 +157,10 @@ def accuracy(nx, ny, steps=500, seed=1):
     conserved state and per field (L1 and Linf), and drift of total mass and energy.
 
     Two FP32 machines are compared with the FP64 one: the product machine (``hydro_f32``:
-    FP32 bulk arithmetic, constants folded exactly, compensated products by the constants whose reciprocal the kernel also multiplies by) and the
-    reference's own FP32 program correctly rounded (``hydro_f32_precise``, which the tests
-    hold bit for bit equal to the FP32 oracle over 500 steps at 512^2)."""
+    FP32 bulk arithmetic, constants folded exactly, compensated products by the constants
+    whose reciprocal the kernel also multiplies by) and the reference's own FP32 program
+    correctly rounded (``hydro_f32_precise``, which the tests hold bit for bit equal to the
+    FP32 oracle over 500 steps at 512^2)."""
     import numpy as np
     from paraiso_amd import variants
     out, totals = {}, {}
