@@ -40,6 +40,10 @@ def _cases():
                                     ["fieldF", "fieldG"]), (64, 64, 64), cyc3))
     out.append((lambda: add_adjoint(linear_wave_om(3, CType.Double), "proceed",
                                     ["fieldF", "fieldG"]), (128, 96, 112), cyc3))
+    out.append((lambda: add_adjoint(linear_wave_om(1, CType.Double), "proceed",
+                                    ["fieldF", "fieldG"]), (1000,), (A.Condition.Cyclic,)))
+    out.append((lambda: add_adjoint(linear_wave_om(2, CType.Double), "proceed",
+                                    ["fieldF", "fieldG"]), (100, 37), cyc2))
     out.append((lambda: add_adjoint(linear_wave_om(3, CType.Float), "proceed",
                                     ["fieldF", "fieldG"]), (64, 64, 64), cyc3))
     for n in [(64, 48), (96, 64), (48, 32), (80, 56), (512, 512)]:

@@ -14,6 +14,7 @@ from .om.dyn import CType
 
 CYC2 = (A.Condition.Cyclic,) * 2
 CYC3 = (A.Condition.Cyclic,) * 3
+CYC1 = (A.Condition.Cyclic,)
 OPEN2 = (A.Condition.Open,) * 2
 
 
@@ -157,6 +158,12 @@ VARIANTS = {
     "wave3d_f64_march": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,
                                                            boundary=CYC3, tile=(32, 8, 32),
                                                            march=1), True),
+    # the reference's own LinearWave is 1-D (Convergence.hs:25-54): 1-D and 2-D machines of
+    # the same program, checked bit for bit
+    "wave1d_f64": (_wave_adj(1, CType.Double), Setup((256,), Language.HIP, boundary=CYC1,
+                                                     tile=(256,), threads=256), True),
+    "wave2d_f64": (_wave_adj(2, CType.Double), Setup((64, 64), Language.HIP, boundary=CYC2,
+                                                     tile=(32, 16), threads=512), True),
     # block tiles with the resident tile loop, whose load phase computes the half step f1
     # from the next tile's prefetched fields, checked bit for bit
     "wave3d_f64_persist": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP,

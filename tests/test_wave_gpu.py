@@ -45,6 +45,31 @@ def test_wave3d_f64_bit_exact_vs_oracle(require_gpu, variant, n):
     assert abs(gpu.get("energy") - ora.get("energy")) <= 1e-12 * abs(ora.get("energy"))
 
 
+@pytest.mark.parametrize("variant,ndim,n", [("wave1d_f64", 1, (1000,)),
+                                            ("wave2d_f64", 2, (100, 37))])
+def test_wave_1d_2d_bit_exact_vs_oracle(require_gpu, variant, ndim, n):
+    """The reference's LinearWave is 1-D (Convergence.hs:34-54, with c = 3.43, dx = 2 pi / n
+    and dt = dx / c); the same program in 1-D and 2-D, one and two steps per launch, bit for
+    bit against the oracle."""
+    print("wave case %s %s" % (variant, n), flush=True)
+    rng = np.random.default_rng(11)
+    f0, g0 = rng.standard_normal(n[::-1]), rng.standard_normal(n[::-1])
+    dx = 2 * np.pi / n[0]
+    for kern, launches in (("proceed", 12), ("proceed_x2", 6)):
+        gpu = variants.machine(variant, n)
+        ora = oracle_machine(lambda: add_adjoint(linear_wave_om(ndim, CType.Double), "proceed",
+                                                 ["fieldF", "fieldG"]), n)
+        for m in (gpu, ora):
+            params(m, 3.43, dx, dx / 3.43)
+            m.set("fieldF", f0)
+            m.set("fieldG", g0)
+        gpu.run(kern, launches)
+        ora.run("proceed", 12)
+        assert np.array_equal(gpu.get("fieldF"), ora.get("fieldF"))
+        assert np.array_equal(gpu.get("fieldG"), ora.get("fieldG"))
+        assert abs(gpu.get("energy") - ora.get("energy")) <= 1e-12 * abs(ora.get("energy"))
+
+
 def test_wave3d_exact_plane_wave(require_gpu):
     n = (32, 16, 8)
     c, dx, dt, steps = 1.0, 1.0, 0.5, 40
