@@ -209,6 +209,9 @@ class MachineEmitter:
         # with march: issue the global loads of step s + 1 into registers before step s
         # computes, so their latency overlaps the step's arithmetic
         self.prefetch = prefetch
+        # register sets of streaming prefetch: the loads of step s + depth issue at step s
+        # (two: the 3-D wave's launches 0.7-1.3% faster than with one, results unchanged)
+        self.prefetch_depth = 2
         # block tiles: a resident grid of workgroups loops over the tiles (see emit_launch)
         self.persist = persist and not march
         self._m = None
@@ -651,27 +654,51 @@ class MachineEmitter:
         inner = la.phases[:-1]
         s0 = min([(ph.box[ma][0] - ph.box[ma][1]) // R for ph in inner] + [0])
         pf = self.prefetch and inner and inner[0].index == 0 and inner[0].nodes
+        # with two register sets the loads of step s + 2 are issued at step s: two steps of
+        # arithmetic to cover their latency (the loop is unrolled twice, the sets alternate)
+        depth = 2 if pf and self.prefetch_depth >= 2 else 1
+        N = Z // R
         w("  // march along axis %d: %d planes per step over %d planes" % (ma, R, Z))
+        n_pf = 0
         if pf:
             ph = inner[0]
             self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
-            self._emit_march_loads(k, la, ph, w, str(s0), declare=True)
-        w("  for (int s = %d; s < %d; ++s) {" % (s0, Z // R))
-        for pi, ph in enumerate(inner):
-            self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
-            if pf and pi == 0:
-                self._emit_march_ring_writes(k, la, ph, w)
-                w("  __syncthreads();")
-                w("  if (s + 1 < %d) {" % (Z // R))
-                self._emit_march_loads(k, la, ph, w, "(s + 1)", declare=False)
-                w("  }")
-                continue
-            self._emit_phase(k, la, ph, w)
-        self._m = {"R": R, "ma": ma, "lo": 0, "hi": 0}
-        w("  if (s >= 0) {")
-        self._emit_phase(k, la, la.phases[-1], w)
-        w("  }")
-        w("  }")
+            n_pf = self._march_points(self._march_box(ph)[2])[2]
+            for d in range(depth):
+                self._emit_march_loads(k, la, ph, w, str(s0 + d), declare=True,
+                                       base=d * n_pf)
+
+        def step_body(base):
+            for pi, ph in enumerate(inner):
+                self._m = {"R": R, "ma": ma, "lo": ph.box[ma][0], "hi": ph.box[ma][1]}
+                if pf and pi == 0:
+                    self._emit_march_ring_writes(k, la, ph, w, base)
+                    w("  __syncthreads();")
+                    w("  if (s + %d < %d) {" % (depth, N))
+                    self._emit_march_loads(k, la, ph, w, "(s + %d)" % depth, declare=False,
+                                           base=base)
+                    w("  }")
+                    continue
+                self._emit_phase(k, la, ph, w)
+            self._m = {"R": R, "ma": ma, "lo": 0, "hi": 0}
+            w("  if (s >= 0) {")
+            self._emit_phase(k, la, la.phases[-1], w)
+            w("  }")
+        if depth == 1:
+            w("  for (int s = %d; s < %d; ++s) {" % (s0, N))
+            step_body(0)
+            w("  }")
+        else:
+            w("  for (int s_ = %d; s_ < %d; s_ += 2) {" % (s0, N))
+            w("  {")
+            w("  const int s = s_;")
+            step_body(0)
+            w("  }")
+            w("  if (s_ + 1 < %d) {" % N)
+            w("  const int s = s_ + 1;")
+            step_body(n_pf)
+            w("  }")
+            w("  }")
         self._m = None
 
     def _march_box(self, ph):
@@ -750,17 +777,18 @@ class MachineEmitter:
         w("    }")
         w("  }")
 
-    def _emit_march_loads(self, k, la, ph, w, step, declare):
+    def _emit_march_loads(self, k, la, ph, w, step, declare, base=0):
         """Global loads of a load-only phase for march step `step` into registers
-        pf<v>_<i> (one per loop point i of the thread), which the next ring write stores."""
+        pf<v>_<base + i> (one per loop point i of the thread), which a later ring write
+        stores."""
         box, W, vol = self._march_box(ph)
         self._emit_prefetch(k, la, ph, w, box, W, self._march_points(vol)[2],
                             declare, True, "for step %s" % step,
                             lambda j: self._march_point(box, W, vol, j, step, w),
-                            lambda guarded: self._march_close(guarded, w))
+                            lambda guarded: self._march_close(guarded, w), base)
 
     def _emit_prefetch(self, k, la, ph, w, box, W, n_it, declare, load, what, open_point,
-                       close_point):
+                       close_point, base=0):
         """Loads of a load-only phase into registers pf<v>_<i>, i the thread's loop point:
         direct addressing for tiles whose reads stay inside the grid (a tile-uniform test),
         wrapped or clamped addressing for the others.  ``declare`` declares the registers
@@ -769,7 +797,7 @@ class MachineEmitter:
         if declare:
             for v in roots:
                 tn = k.nodes[v].ctype.c_name
-                w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, i, tn)
+                w("  %s %s;" % (tn, ", ".join("pf%d_%d = (%s)0" % (v, base + i, tn)
                                               for i in range(n_it))))
         if not load:
             return
@@ -800,13 +828,13 @@ class MachineEmitter:
                     mt = re.match(r"const \w+ n(\d+)_0(?:_0)* = (.*)$", ln)
                     if not mt or int(mt.group(1)) not in roots:
                         raise ValueError("prefetch expects plain loads, got: %s" % ln)
-                    w("      pf%s_%d = %s" % (mt.group(1), i, mt.group(2)))
+                    w("      pf%s_%d = %s" % (mt.group(1), base + i, mt.group(2)))
                 close_point(guarded)
         if conds is not None:
             w("  }")
 
-    def _emit_march_ring_writes(self, k, la, ph, w):
-        """Store the registers loaded for this step into the value rings."""
+    def _emit_march_ring_writes(self, k, la, ph, w, base=0):
+        """Store the registers loaded for this step (pf<v>_<base + i>) into the value rings."""
         box, W, vol = self._march_box(ph)
         n_it = self._march_points(vol)[2]
         w("  // phase %d: values %s into their rings (loaded one step ahead)"
@@ -817,12 +845,12 @@ class MachineEmitter:
             guarded = self._march_point(box, W, vol, i, "s", w)
             if computed:
                 # values computed point by point from the prefetched fields
-                self._pf = i
+                self._pf = base + i
                 for ln in self._ctx_body(k, la, list(ph.nodes), box, W, False):
                     w("      " + ln)
                 self._pf = None
             for v in ph.nodes:
-                src = ("n%d_%s" % (v, zero)) if computed else ("pf%d_%d" % (v, i))
+                src = ("n%d_%s" % (v, zero)) if computed else ("pf%d_%d" % (v, base + i))
                 w("      s%d[%s] = %s;" % (v, self._lds_self(la, v, W), src))
             self._march_close(guarded, w)
 
