@@ -68,6 +68,13 @@ CANDIDATE_SETS = {
                       ((32, 32), 512), ((64, 16), 1024), ((32, 32), 1024), ((64, 32), 1024),
                       ((128, 8), 1024), ((32, 8), 256, 24), ((32, 8), 256, 96),
                       ((64, 8), 256, 96)],
+    # the same with the join of shared expansions (a sixth element), beside the forms the
+    # staged Hydro adjoint had tuned without it
+    "per_kernel_join": [((64, 8), 512, 48, 0, None, True), ((32, 16), 512, 48, 0, None, True),
+                        ((32, 32), 512, 48, 0, None, True), ((16, 32), 512, 48, 0, None, True),
+                        ((64, 16), 1024, 48, 0, None, True),
+                        ((32, 32), 1024, 48, 0, None, True), ((64, 8), 512), ((32, 32), 512),
+                        ((64, 16), 1024), ((32, 32), 1024)],
     # timed with --depths: each tile at every number of steps fused per launch
     "life_depth": [((64, 32), 1024), ((128, 32), 1024), ((64, 64), 1024), ((32, 64), 1024),
                    ((128, 16), 512), ((64, 16), 512)],
@@ -93,27 +100,31 @@ def tuned() -> Dict:
 
 
 def candidate_name(variant: str, tile, threads: int = 256, mat_gain: int = 48,
-                   march: int = 0, persist=None) -> str:
+                   march: int = 0, persist=None, join: bool = False) -> str:
     g = "" if mat_gain == 48 else "_g%d" % mat_gain
     mr = "_m%d" % march if march else ""
     pr = "" if persist is None else ("_p" if persist else "_np")
-    return "%s__t%s_n%d%s%s%s" % (variant, "x".join(str(t) for t in tile), threads, g, mr, pr)
+    jn = "_j" if join else ""
+    return "%s__t%s_n%d%s%s%s%s" % (variant, "x".join(str(t) for t in tile), threads, g, mr, pr,
+                                    jn)
 
 
 def _unpack(c):
-    """(tile, threads[, mat_gain[, march[, persist]]]) -> (tile, threads, mat_gain, march,
-    persist); persist None keeps the variant's own choice of the resident tile loop."""
+    """(tile, threads[, mat_gain[, march[, persist[, join]]]]) -> (tile, threads, mat_gain,
+    march, persist, join); persist None keeps the variant's own choice of the resident tile
+    loop."""
     return (tuple(c[0]), int(c[1]), int(c[2]) if len(c) > 2 else 48,
-            int(c[3]) if len(c) > 3 else 0, bool(c[4]) if len(c) > 4 else None)
+            int(c[3]) if len(c) > 3 else 0, None if len(c) <= 4 or c[4] is None else bool(c[4]),
+            bool(c[5]) if len(c) > 5 else False)
 
 
 def _build_candidate(variant: str, c):
     from . import variants
     from .generator import build_machine
     mk, setup, precise = variants.VARIANTS[variant]
-    tile, nt, gain, march, persist = _unpack(c)
-    name = candidate_name(variant, tile, nt, gain, march, persist)
-    form = dict(tile=tile, threads=nt, mat_gain=gain, march=march)
+    tile, nt, gain, march, persist, join = _unpack(c)
+    name = candidate_name(variant, tile, nt, gain, march, persist, join)
+    form = dict(tile=tile, threads=nt, mat_gain=gain, march=march, join=join)
     if persist is not None:
         form["persist"] = persist
     try:
@@ -178,13 +189,13 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
     cands = candidates or CANDIDATES[setup.ndim]
     results = []
     for c in cands:
-        tile, nt, gain, march, persist = _unpack(c)
-        name = candidate_name(libs_of or variant, tile, nt, gain, march, persist)
+        tile, nt, gain, march, persist, join = _unpack(c)
+        name = candidate_name(libs_of or variant, tile, nt, gain, march, persist, join)
         so = os.path.join(GEN_DIR, name, "lib%s.so" % name)
         if not os.path.exists(so):
             continue
-        print("tune %s tile=%s threads=%d mat_gain=%d march=%d persist=%s"
-              % (variant, tile, nt, gain, march, persist), flush=True)
+        print("tune %s tile=%s threads=%d mat_gain=%d march=%d persist=%s join=%s"
+              % (variant, tile, nt, gain, march, persist, join), flush=True)
         if batch:
             sizes = [tuple(int(x) for x in sz) for sz in inputs.batched_sizes(batch, 2024)]
             m = Batch(so, sizes)
@@ -204,7 +215,7 @@ def measure(variant: str, size, steps: int = 50, warmup: int = 5, candidates=Non
             launches = max(1, steps // per)
             ms = m.run_timed(kern, launches)
             rec = {"tile": list(tile), "threads": nt, "mat_gain": gain, "march": march,
-                   "persist": persist, "kernel": kern, "steps_per_launch": per,
+                   "persist": persist, "join": join, "kernel": kern, "steps_per_launch": per,
                    "ms_per_step": sum(ms) / launches / per,
                    "launch_ms": [x / launches for x in ms]}
             if kernels:
@@ -235,7 +246,8 @@ def alloc_candidates(variant: str, kernel: str, count: int = 8) -> List[Tuple[st
     f = setup.form_of(kernel)
     lk = lower_kernel(om, ki, tuple(True for _ in range(om.ndim)), exact=not precise)
     sch = schedule_kernel(lk, tuple(f["tile"]), lds_budget(f["threads"]), f["mat_gain"],
-                          f["march"], f.get("manifest", ()), f.get("delayed", ()))
+                          f["march"], f.get("manifest", ()), f.get("delayed", ()),
+                          join=f.get("join", False))
     mat, expanded = [], []
     for la in sch.launches:
         for gain, v, in_m in la.choices:
@@ -369,8 +381,11 @@ def select_per_kernel(path: str) -> Dict:
             forms[k] = {"tile": best["tile"], "threads": best["threads"],
                         "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
                         "ms": round(best["per_kernel_ms"][k], 5)}
-            table[k] = {"%s_n%d_g%d_m%d" % ("x".join(map(str, r["tile"])), r["threads"],
-                                            r.get("mat_gain", 48), r.get("march", 0)):
+            if best.get("join"):
+                forms[k]["join"] = True
+            table[k] = {"%s_n%d_g%d_m%d%s" % ("x".join(map(str, r["tile"])), r["threads"],
+                                              r.get("mat_gain", 48), r.get("march", 0),
+                                              "_j" if r.get("join") else ""):
                         round(r["per_kernel_ms"][k], 5) for r in res}
         t[variant] = {"kernel_forms": forms, "size": rec["size"], "steps": rec["steps"],
                       "candidates_per_kernel": table}
@@ -390,6 +405,8 @@ def select(path: str) -> Dict:
         t[variant] = dict(keep)
         if best.get("persist") is not None:
             t[variant]["persist"] = best["persist"]
+        if best.get("join"):
+            t[variant]["join"] = True
         t[variant].update({"tile": best["tile"], "threads": best["threads"],
                       "mat_gain": best.get("mat_gain", 48), "march": best.get("march", 0),
                       "kernel": best.get("kernel"), "steps_per_launch": best.get("steps_per_launch", 1),
@@ -400,8 +417,9 @@ def select(path: str) -> Dict:
                                                              r.get("march", 0),
                                                              "" if r.get("persist") is None
                                                              else ("_p" if r["persist"] else "_np"),
-                                                             "" if r.get("steps_per_launch", 1) == 1
-                                                             else "_k%d" % r["steps_per_launch"]):
+                                                             ("" if r.get("steps_per_launch", 1) == 1
+                                                              else "_k%d" % r["steps_per_launch"])
+                                                             + ("_j" if r.get("join") else "")):
                                      round(r["ms_per_step"], 5) for r in rec["results"]}})
     with open(TUNED, "w") as f:
         json.dump(t, f, indent=1, sort_keys=True)

@@ -56,7 +56,8 @@ def _hip_sources(setup: Setup, om: OM, precise: bool = True) -> Tuple[Plan, str,
         f["tile"] = tuple(f["tile"]) if f["tile"] else tile
         forms.append(f)
         scheds.append(schedule_kernel(k, f["tile"], lds_budget(f["threads"]), f["mat_gain"],
-                                      f["march"], f.get("manifest", ()), f.get("delayed", ())))
+                                      f["march"], f.get("manifest", ()), f.get("delayed", ()),
+                                      join=f.get("join", False)))
     src = MachineEmitter(om_opt.name, om_opt.setup.static_values, om_opt.ndim, scheds, tile,
                          fast_fp32=not precise, batched=setup.batched,
                          threads=setup.threads, open_axes=tuple(not c for c in cyclic),

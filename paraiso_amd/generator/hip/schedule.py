@@ -32,12 +32,15 @@ from .ir import LKernel, LNode
 
 OUT = -1                      # the store phase context
 _FORCE = (frozenset(), frozenset())   # per-node Manifest/Delayed overrides of the kernel scheduled
+_JOIN = False                 # the kernel scheduled joins shared expansions into earlier phases
 SCHED_DEBUG = False
 CUT_GAIN = 2.0               # unlimited materialisation (with cuts) must save this factor of work
 MAT_GAIN = 48                 # per-point operations a value must save to earn an LDS array
 LDS_BUDGET = 64 * 1024        # bytes per 256-thread workgroup; larger launches are cut
 LDS_MAX = 160 * 1024          # LDS of one CU (gfx950)
 RED_SCRATCH = 16 * 8          # LDS a kernel adds per Reduce: one 8-byte slot per wave (<= 16)
+JOIN_SHARE = 0.5              # share of a value's expansion an earlier phase must already evaluate
+JOIN_GAIN = 0.1               # share of a launch's work a joined group must save
 
 
 def lds_cap(reduces) -> int:
@@ -119,16 +122,24 @@ def _neg(v):
 
 def schedule_kernel(k: LKernel, tile: Tuple[int, ...], budget: int = LDS_BUDGET,
                     mat_gain: int = MAT_GAIN, march: int = 0, force_manifest=(),
-                    force_delayed=()) -> KernelSchedule:
+                    force_delayed=(), join: bool = False) -> KernelSchedule:
     """``march`` > 0: workgroups stream along the last axis, ``march`` planes per step, and
     keep each materialised value as a ring of planes (see the emitter).
 
     ``force_manifest`` / ``force_delayed``: values (ids of the lowered kernel) whose
     Manifest/Delayed choice is fixed rather than left to the ``mat_gain`` threshold -- the
     per-node ``AllocationChoice`` of the reference (DecideAllocation.hs:41-43), which its
-    genetic tuner flipped (Tuning/Genetic.hs:138-146) and the autotuner here searches."""
-    global _FORCE
+    genetic tuner flipped (Tuning/Genetic.hs:138-146) and the autotuner here searches.
+
+    ``join``: after the greedy, values that a later phase evaluates in registers and an
+    earlier phase mostly evaluates already are materialised there as one group (see
+    ``join_phases``), and a launch may take the whole CU's LDS: a launch whose registers
+    already hold one workgroup per CU loses no occupancy by it."""
+    global _FORCE, _JOIN
     _FORCE = (frozenset(force_manifest), frozenset(force_delayed))
+    _JOIN = bool(join)
+    if _JOIN:
+        budget = LDS_MAX
     nd = k.ndim
     zero = tuple(0 for _ in range(nd))
     nodes = k.nodes
@@ -538,6 +549,82 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
                 total += len(curs) * vol.get(ctx, 0)
         return total
 
+    def join_phases(M):
+        """Materialise, as one group, the values a later phase evaluates in registers whose
+        expansion an earlier phase mostly evaluates already (at the points it covers).
+
+        A reverse sweep has many such values: the cotangents of a face's left and right
+        states, each read once by the cell update, expand into the face's whole forward
+        computation, which the phase that materialises the face's other cotangents also
+        evaluates.  One of them alone saves little, as its siblings still need the shared
+        expansion; together the later phase keeps none of it.  The group is the maximal
+        such values (no other member reads them); it is kept when ``work`` falls by
+        JOIN_GAIN and the launch still fits the CU's LDS."""
+        phase, out_phase, members, lc = analyse(M)
+        cone_sets: Dict[int, Set[int]] = {}
+        reads: Dict[int, Set[int]] = {}        # materialised values each expansion reads
+        for v in computed:
+            n = nodes[v]
+            s_ = {v} if n.op == "arith" else set()
+            r_: Set[int] = set()
+            for a in n.args:
+                if a in M:
+                    r_.add(a)
+                elif a in cone_sets:
+                    s_ |= cone_sets[a]
+                    r_ |= reads[a]
+            cone_sets[v] = s_
+            reads[v] = r_
+        evaluated: Dict[int, Set[int]] = {}
+        for x, ctxs in lc.items():
+            for ctx in ctxs:
+                if ctx != OUT and ctx > 0:
+                    evaluated.setdefault(ctx, set()).add(x)
+        cands: Dict[int, int] = {}
+        for v in computed:
+            if (v in M or v in target_set or v in force_d or nodes[v].op != "arith"
+                    or not lc.get(v)):
+                continue
+            cs = cone_sets[v]
+            if len(cs) < mat_gain:
+                continue
+            best = None
+            for P, X in evaluated.items():
+                # P must be able to compute v: everything v reads comes from earlier phases
+                if set(lc[v]) <= {P} or any(phase[u] >= P for u in reads[v]):
+                    continue
+                share = len(cs & X) / len(cs)
+                if share >= JOIN_SHARE and (best is None or share > best[0]):
+                    best = (share, P)
+            if best is not None:
+                cands[v] = best[1]
+        dominated = set()
+        for u, P in cands.items():
+            dominated |= set(x for x in cone_sets[u] if x != u and cands.get(x) == P)
+        group = set(cands) - dominated
+        if not group:
+            return M
+        trial = set(M) | group
+        w0, w1 = work(M), work(trial)
+        top1 = layout(trial)[6]
+        if SCHED_DEBUG:
+            print("join L%d: %d values, work %d -> %d, LDS %d" % (L, len(group), w0, w1, top1))
+        if top1 > lds_cap(reduces):
+            # over the CU's LDS: leave out the members whose absence costs least work
+            cost = sorted((work(trial - {v}) - w1, v) for v in group)
+            for _, v in cost:
+                trial.discard(v)
+                top1 = layout(trial)[6]
+                if top1 <= lds_cap(reduces):
+                    break
+            w1 = work(trial)
+            if SCHED_DEBUG:
+                print("join L%d: %d values kept, work %d, LDS %d"
+                      % (L, len(trial - set(M)), w1, top1))
+        if w1 <= (1 - JOIN_GAIN) * w0 and top1 <= lds_cap(reduces):
+            return trial
+        return M
+
     M = free_fields(M)
     M_fit = greedy(set(M), budget, 64)
     if budget is not None and allow_free:
@@ -575,6 +662,8 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
             uses = sum(len(c) for c in lc.get(v, {}).values())
             if uses <= 1:
                 M.discard(v)
+    if _JOIN:
+        M = join_phases(M)
     phase, out_phase, members, lc, phases, lds, top = layout(M)
     if top > lds_cap(reduces):
         # the tiles the launch starts from overflow LDS (a launch behind a cut may read many

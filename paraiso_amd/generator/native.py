@@ -44,8 +44,12 @@ class Setup:
     # block tiles: launch as many workgroups as fit on the device at once and let each loop
     # over tiles, loading the next tile's fields while it computes the current one
     persist: bool = False
+    # after the greedy LDS choice, also compute in an earlier phase, as one group, the values
+    # a later phase evaluates in registers from mostly the same expansion, and let a launch
+    # take the whole CU's LDS (for launches whose registers allow one workgroup per CU)
+    join: bool = False
     # per OM kernel, a launch form overriding the above: ((kernel name, {"tile": ...,
-    # "threads": ..., "persist": ..., "march": ..., "mat_gain": ...}), ...)
+    # "threads": ..., "persist": ..., "march": ..., "mat_gain": ..., "join": ...}), ...)
     kernel_forms: Tuple[Tuple[str, dict], ...] = ()
 
     def __post_init__(self):
@@ -68,7 +72,7 @@ class Setup:
     def form_of(self, kernel: str) -> dict:
         """The launch form of one OM kernel: the machine's, with the kernel's overrides."""
         f = {"tile": self.tile, "threads": self.threads, "persist": self.persist,
-             "march": self.march, "mat_gain": self.mat_gain}
+             "march": self.march, "mat_gain": self.mat_gain, "join": self.join}
         for name, over in self.kernel_forms:
             if name == kernel:
                 f.update(over)

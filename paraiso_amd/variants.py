@@ -144,6 +144,14 @@ VARIANTS = {
                             Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 8)), False),
     "hydro_adj2_f64": (_hydro_adj_staged(CType.Double),
                        Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 8)), True),
+    # the same with the adjoint kernels' shared expansions joined into earlier phases
+    # (correctly rounded, so bit-identical to the above)
+    "hydro_adj2_f64_join": (_hydro_adj_staged(CType.Double),
+                            Setup((64, 64), Language.HIP, boundary=CYC2, tile=(32, 16),
+                                  threads=512,
+                                  kernel_forms=(("proceed_stage1_adjoint", {"join": True}),
+                                                ("proceed_stage2_adjoint", {"join": True}))),
+                            True),
     "wave3d_f32": (_wave_adj(3, CType.Float), Setup((64, 64, 64), Language.HIP, boundary=CYC3), False),
     "wave3d_f64": (_wave_adj(3, CType.Double), Setup((64, 64, 64), Language.HIP, boundary=CYC3), True),
     # streaming along the last axis (rings of planes in LDS), checked bit for bit
@@ -191,7 +199,8 @@ def setup_of(name: str):
     if rec is not None and "tile" in rec:
         setup = setup.with_(tile=tuple(rec["tile"]), threads=int(rec.get("threads", 256)),
                             mat_gain=int(rec.get("mat_gain", 48)), march=int(rec.get("march", 0)),
-                            persist=bool(rec.get("persist", setup.persist)))
+                            persist=bool(rec.get("persist", setup.persist)),
+                            join=bool(rec.get("join", setup.join)))
     if rec is not None and "kernel_forms" in rec:
         forms = []
         for k, f in sorted(rec["kernel_forms"].items()):
@@ -200,7 +209,8 @@ def setup_of(name: str):
                 form.update({"tile": tuple(f["tile"]), "threads": int(f["threads"]),
                              "mat_gain": int(f.get("mat_gain", 48)),
                              "march": int(f.get("march", 0)),
-                             "persist": bool(f.get("persist", False))})
+                             "persist": bool(f.get("persist", False)),
+                             "join": bool(f.get("join", False))})
             # the per-node Manifest/Delayed choices the autotuner kept
             for key in ("manifest", "delayed"):
                 if f.get(key):

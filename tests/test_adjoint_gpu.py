@@ -194,3 +194,23 @@ def test_rollout_with_strided_grad_and_transposed_input(require_gpu):
         err = np.abs(xi.grad.cpu().numpy() - ti.grad.numpy()).max() / max(
             np.abs(ti.grad.numpy()).max(), 1e-300)
         assert err < 1e-10, (f, err)
+
+
+def test_joined_adjoint_is_bit_identical(require_gpu):
+    """The adjoint kernels with their shared expansions joined into earlier phases compute
+    the same correctly rounded operations on the same values: the gradient is bit for bit
+    that of the schedule without the join."""
+    nx, ny, dt, nsteps, every = 48, 40, 1e-3, 6, 3
+    print("joined staged adjoint %dx%d" % (nx, ny), flush=True)
+    state = generic_state(nx, ny, 5)
+    grads = []
+    for variant in ("hydro_adj2_f64", "hydro_adj2_f64_join"):
+        m = machine(variant, nx, ny, dt)
+        fields = [torch.tensor(a, dtype=torch.float64, device="cuda", requires_grad=True)
+                  for a in state]
+        out = rollout(m, F, fields, nsteps, every, **STAGED)
+        _, mask = HT.kinetic_energy_in_square(*[o.detach().cpu() for o in out[:3]])
+        objective(out, mask.cuda().to(torch.float64)).backward()
+        grads.append([x.grad.cpu().numpy() for x in fields])
+    for f, a, b in zip(F, *grads):
+        assert np.array_equal(a, b), f

@@ -131,10 +131,47 @@ def test_resident_loop_is_a_candidate_dimension():
     """A candidate may fix the resident tile loop (a fifth element); without it the variant's
     own choice stands and the candidate keeps its earlier name."""
     from paraiso_amd import autotune
-    assert autotune._unpack(((64, 32), 1024)) == ((64, 32), 1024, 48, 0, None)
+    assert autotune._unpack(((64, 32), 1024)) == ((64, 32), 1024, 48, 0, None, False)
     assert autotune._unpack(((64, 32), 1024, 48, 0, True))[4] is True
     assert autotune.candidate_name("v", (64, 32), 1024) == "v__t64x32_n1024"
     assert autotune.candidate_name("v", (64, 32), 1024, 48, 0, True) == "v__t64x32_n1024_p"
     assert autotune.candidate_name("v", (64, 32), 1024, 48, 0, False) == "v__t64x32_n1024_np"
     for c in autotune.CANDIDATE_SETS["headline_persist"]:
         assert autotune._unpack(c)[4] is not None
+
+
+def _stage2_adjoint_f32():
+    from paraiso_amd import variants
+    om = variants._hydro_adj_staged(CType.Float)()
+    plan = translate(Setup((64, 64), Language.HIP, boundary=CYC2), om)
+    ki = [k.name for k in plan.om.kernels].index("proceed_stage2_adjoint")
+    return lower_kernel(plan.om, ki, (True, True), exact=True)
+
+
+def test_join_computes_shared_expansions_once():
+    """The stage-2 Hydro adjoint: the cell update reads each face's left- and right-state
+    cotangents once, and each of them expands into the face's forward HLLC work, which the
+    phase that materialises the face's other cotangents evaluates already.  With ``join``
+    they are computed there, as one group, and read from LDS: the launch stays whole within
+    the CU's LDS and the store phase keeps under half of its register expansion."""
+    from paraiso_amd.generator.hip.schedule import OUT
+    k = _stage2_adjoint_f32()
+    nodes = k.nodes
+
+    def out_evals(la):
+        return sum(len(c) for v, ctxs in la.lc.items() if nodes[v].op == "arith"
+                   for ctx, c in ctxs.items() if ctx == OUT)
+    cut = schedule_kernel(k, (64, 8), lds_budget(512), 48)
+    full = schedule_kernel(k, (64, 8), LDS_MAX, 48)
+    joined = schedule_kernel(k, (64, 8), lds_budget(512), 48, join=True)
+    assert len(cut.launches) == 2 and len(full.launches) == 1
+    assert len(joined.launches) == 1 and joined.launches[0].lds_bytes <= LDS_MAX
+    fl, jl = full.launches[0], joined.launches[0]
+    assert out_evals(jl) < 0.5 * out_evals(fl)
+    group = jl.mat - fl.mat
+    phase = {v: ph.index for ph in jl.phases[:-1] for v in ph.nodes}
+    assert len(group) >= 16
+    assert all(phase[v] < jl.phases[-1].index for v in group)
+    # without join the schedule is the one the default budget gives
+    again = schedule_kernel(k, (64, 8), lds_budget(512), 48, join=False)
+    assert [la.mat for la in again.launches] == [la.mat for la in cut.launches]
