@@ -74,7 +74,9 @@ CANDIDATE_SETS = {
                         ((32, 32), 512, 48, 0, None, True), ((16, 32), 512, 48, 0, None, True),
                         ((64, 16), 1024, 48, 0, None, True),
                         ((32, 32), 1024, 48, 0, None, True), ((64, 8), 512), ((32, 32), 512),
-                        ((64, 16), 1024), ((32, 32), 1024)],
+                        ((64, 16), 1024), ((32, 32), 1024), ((32, 16), 512, 48, 0, True, True),
+                        ((32, 32), 1024, 48, 0, True, True), ((64, 8), 512, 48, 0, True, True),
+                        ((32, 16), 512, 24, 0, None, True), ((32, 16), 512, 96, 0, None, True)],
     # timed with --depths: each tile at every number of steps fused per launch
     "life_depth": [((64, 32), 1024), ((128, 32), 1024), ((64, 64), 1024), ((32, 64), 1024),
                    ((128, 16), 512), ((64, 16), 512)],
@@ -383,9 +385,13 @@ def select_per_kernel(path: str) -> Dict:
                         "ms": round(best["per_kernel_ms"][k], 5)}
             if best.get("join"):
                 forms[k]["join"] = True
-            table[k] = {"%s_n%d_g%d_m%d%s" % ("x".join(map(str, r["tile"])), r["threads"],
-                                              r.get("mat_gain", 48), r.get("march", 0),
-                                              "_j" if r.get("join") else ""):
+            if best.get("persist") is not None:
+                forms[k]["persist"] = best["persist"]
+            table[k] = {"%s_n%d_g%d_m%d%s%s" % ("x".join(map(str, r["tile"])), r["threads"],
+                                                r.get("mat_gain", 48), r.get("march", 0),
+                                                "" if r.get("persist") is None
+                                                else ("_p" if r["persist"] else "_np"),
+                                                "_j" if r.get("join") else ""):
                         round(r["per_kernel_ms"][k], 5) for r in res}
         t[variant] = {"kernel_forms": forms, "size": rec["size"], "steps": rec["steps"],
                       "candidates_per_kernel": table}
