@@ -175,3 +175,32 @@ def test_join_computes_shared_expansions_once():
     # without join the schedule is the one the default budget gives
     again = schedule_kernel(k, (64, 8), lds_budget(512), 48, join=False)
     assert [la.mat for la in again.launches] == [la.mat for la in cut.launches]
+
+
+def test_per_kernel_selection_records_join_and_resident_loop(tmp_path, monkeypatch):
+    """select --per-kernel keeps, per OM kernel, the fastest candidate's whole form: its
+    join and resident-loop choices too, and setup_of hands them to the generator."""
+    import json
+    from paraiso_amd import autotune, variants
+    tuned = tmp_path / "tuned.json"
+    tuned.write_text("{}")
+    monkeypatch.setattr(autotune, "TUNED", str(tuned))
+    res = [{"tile": [64, 8], "threads": 512, "mat_gain": 48, "march": 0, "persist": None,
+            "join": False, "per_kernel_ms": {"a": 2.0, "b": 1.0}},
+           {"tile": [32, 16], "threads": 512, "mat_gain": 48, "march": 0, "persist": True,
+            "join": True, "per_kernel_ms": {"a": 1.0, "b": 3.0}}]
+    runs = tmp_path / "runs.json"
+    runs.write_text(json.dumps({"hydro_adj2_f32_fast": {"size": [512, 512], "steps": 40,
+                                                        "results": res}}))
+    t = autotune.select_per_kernel(str(runs))
+    forms = t["hydro_adj2_f32_fast"]["kernel_forms"]
+    assert forms["a"]["join"] is True and forms["a"]["persist"] is True
+    assert forms["a"]["tile"] == [32, 16]
+    assert "join" not in forms["b"] and "persist" not in forms["b"]
+    assert set(t["hydro_adj2_f32_fast"]["candidates_per_kernel"]["a"]) == {
+        "64x8_n512_g48_m0", "32x16_n512_g48_m0_p_j"}
+    monkeypatch.setattr(autotune, "tuned", lambda: json.loads(tuned.read_text()))
+    setup = variants.setup_of("hydro_adj2_f32_fast")
+    fa, fb = setup.form_of("a"), setup.form_of("b")
+    assert fa["join"] is True and fa["persist"] is True and fa["tile"] == (32, 16)
+    assert fb["join"] is False and fb["tile"] == (64, 8)
