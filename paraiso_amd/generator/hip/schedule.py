@@ -550,16 +550,19 @@ def _schedule_launch(k: LKernel, L: int, targets, stores, reduces, tail, commit,
         return total
 
     def join_phases(M):
-        """Materialise, as one group, the values a later phase evaluates in registers whose
-        expansion an earlier phase mostly evaluates already (at the points it covers).
+        """Materialise, as one group, values that a later phase evaluates in registers and
+        that an earlier phase P could compute (they read only values of phases before P),
+        when at least JOIN_SHARE of their expansion is values P evaluates already.
 
         A reverse sweep has many such values: the cotangents of a face's left and right
         states, each read once by the cell update, expand into the face's whole forward
         computation, which the phase that materialises the face's other cotangents also
         evaluates.  One of them alone saves little, as its siblings still need the shared
         expansion; together the later phase keeps none of it.  The group is the maximal
-        such values (no other member reads them); it is kept when ``work`` falls by
-        JOIN_GAIN and the launch still fits the CU's LDS."""
+        such values (none lies in another one's expansion).  ``work``, which counts every
+        evaluation over the phases' boxes, decides: the group is kept when it falls by
+        JOIN_GAIN, after leaving out the members whose absence costs least until the launch
+        fits the CU's LDS."""
         phase, out_phase, members, lc = analyse(M)
         cone_sets: Dict[int, Set[int]] = {}
         reads: Dict[int, Set[int]] = {}        # materialised values each expansion reads
