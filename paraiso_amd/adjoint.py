@@ -31,9 +31,9 @@ from typing import Dict, List, Sequence
 
 from .om import builder as B
 from .om.arith import Op
-from .om.dyn import CType, Realm
-from .om.graph import (OM, Arith, Broadcast, Imm, Load, LoadIndex, LoadSize, Named, Reduce,
-                       Shift, Store)
+from .om.dyn import Realm
+from .om.graph import (OM, Arith, Broadcast, Imm, Load, LoadIndex, LoadSize, Reduce, Shift,
+                       Store)
 
 
 def _replay_value(g, v, mapped, statics):

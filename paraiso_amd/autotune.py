@@ -27,7 +27,6 @@ import json
 import os
 import subprocess
 import sys
-import time
 from typing import Dict, List, Tuple
 
 HERE = os.path.dirname(os.path.abspath(__file__))

@@ -12,7 +12,7 @@ from __future__ import annotations
 import hashlib
 import os
 import subprocess
-from typing import List, Optional, Tuple
+from typing import List, Tuple
 
 from .. import annotation as A
 from ..om.graph import OM, Reduce, Store

@@ -31,7 +31,7 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 
 from ... import annotation as A
-from ...om.arith import COMPARISONS, Op, ReduceOp
+from ...om.arith import COMPARISONS, Op
 from ...om.dyn import CType, Realm, fraction_to_float32, fraction_to_float64
 from ...om.graph import (Arith, Broadcast, Graph, Imm, Load, LoadIndex, LoadSize,
                          Reduce, Shift, Store)

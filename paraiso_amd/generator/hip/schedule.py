@@ -26,7 +26,6 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Set, Tuple
 
-from ...om.arith import Op
 from ...om.dyn import CType, Realm
 from .ir import LKernel, LNode
 

@@ -6,7 +6,7 @@ read as a launch hint (workgroup tile) by the HIP generator.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field, replace
+from dataclasses import dataclass, replace
 from enum import Enum
 from typing import Optional, Tuple
 

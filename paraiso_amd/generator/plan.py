@@ -10,15 +10,13 @@ Restates ``Language/Paraiso/Generator/OMTrans.hs:39-206`` and
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Tuple
 
 from .. import annotation as A
 from ..om.dyn import DynValue, Realm
 from ..om.graph import OM, Store
 from ..optimization import optimize
-from ..optimization.allocation import allocation_of
-from ..optimization.boundary import valid_of
 from .native import Setup
 
 

@@ -31,9 +31,8 @@ from __future__ import annotations
 
 from fractions import Fraction
 
-from .. import annotation as A
 from ..om import builder as B
-from ..om.arith import Op, ReduceOp
+from ..om.arith import ReduceOp
 from ..om.dyn import CType, Realm
 
 NDIM = 2

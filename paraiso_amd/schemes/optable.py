@@ -14,7 +14,6 @@ from __future__ import annotations
 from fractions import Fraction
 
 from ..om import builder as B
-from ..om.arith import Op
 from ..om.dyn import CType, Realm
 
 POW_EXPONENT = Fraction(17, 10)

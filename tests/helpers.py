@@ -1,5 +1,4 @@
 """Shared helpers for the parity tests: oracle machines for the current programs."""
-import functools
 import os
 import sys
 

@@ -13,7 +13,7 @@ from paraiso_amd.generator.plan import translate
 from paraiso_amd.om import builder as B
 from paraiso_amd.om.arith import Op, ReduceOp
 from paraiso_amd.om.dyn import CType, Realm
-from paraiso_amd.om.graph import Arith, Imm, Load, Shift, Store
+from paraiso_amd.om.graph import Arith, Imm, Load, Store
 from paraiso_amd.om.pretty import pretty_print
 from paraiso_amd.optimization import optimize
 from paraiso_amd.schemes.hydro import hydro_om, hydro_om_legacy
