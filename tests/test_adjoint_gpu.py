@@ -223,11 +223,12 @@ def test_staged_adjoint_at_the_benchmark_size(require_gpu):
     The FP32 product machine's density gradient against the FP64 machine's (relative L1
     3.4e-5 measured, bound 1e-4); then the FP64
     gradient against central differences of the FP64 forward along 16 seeded random
-    directions of the initial density.  The KH state has exact jumps (density and
-    velocity0 across the band edges), where the minmod limiter sits on its kink, and 200
-    steps over 512^2 cells cross limiter and HLLC switches: central differences average the
-    two one-sided derivatives there while the adjoint takes one branch, so they agree to
-    about 1e-3 here (measured 1.2e-3), not to the 1e-5 of the small smooth states above."""
+    directions of the initial density.  200 steps over 512^2 cells cross limiter and HLLC
+    switches (and the KH state's exact jumps put the minmod limiter on its kink): where a
+    perturbed run crosses one, central differences average the two one-sided derivatives
+    while the adjoint takes one branch.  The share of directions that cross one falls with
+    the step: over these 16 directions the worst gap was 1.2e-3 at a step of 1e-7 and
+    2.8e-4 at 1e-8, the step used here (the median of the first eight: 2.2e-5 and 4.5e-6)."""
     n, dt, nsteps, every = 512, 1e-4, 200, 20
     print("staged adjoint %d^2 x %d" % (n, nsteps), flush=True)
     state = inputs.kh_init(n, n, 7)
@@ -245,7 +246,7 @@ def test_staged_adjoint_at_the_benchmark_size(require_gpu):
     err = float((g32 - g64).abs().sum() / g64.abs().sum())
     print("FP32 density gradient: relative L1 %.3g against FP64" % err, flush=True)
     rng = np.random.default_rng(17)
-    eps, worst = 1e-7, 0.0
+    eps, worst = 1e-8, 0.0
     base = [xi.detach() for xi in x]
     for k in range(16):
         d = torch.tensor(rng.standard_normal((n, n)), device="cuda")
@@ -259,7 +260,7 @@ def test_staged_adjoint_at_the_benchmark_size(require_gpu):
         worst = max(worst, abs(fd - an) / abs(an))
     print("central differences at %d^2: worst relative gap %.3g" % (n, worst), flush=True)
     assert err < 1e-4, err
-    assert worst < 5e-3, worst
+    assert worst < 5e-4, worst
 
 
 def test_staged_adjoint_full_grid_matches_cpu_reference(require_gpu):
