@@ -172,9 +172,6 @@ def test_join_computes_shared_expansions_once():
     phase = {v: ph.index for ph in jl.phases[:-1] for v in ph.nodes}
     assert len(group) >= 16
     assert all(phase[v] < jl.phases[-1].index for v in group)
-    # without join the schedule is the one the default budget gives
-    again = schedule_kernel(k, (64, 8), lds_budget(512), 48, join=False)
-    assert [la.mat for la in again.launches] == [la.mat for la in cut.launches]
 
 
 def test_per_kernel_selection_records_join_and_resident_loop(tmp_path, monkeypatch):
